@@ -1,0 +1,165 @@
+/*
+ * lgca.h -- C ABI of the MI355X lattice-gas engine (liblgca_hip.so).
+ *
+ * Plain C types only: an opaque handle, plain pointers and sizes.  Every entry
+ * point returns an lgca_status (0 = success); the text of the last failure on
+ * a handle is available from lgca_last_error().  Host pointers are ordinary
+ * host memory; entry points that name a "device" pointer take memory on the
+ * handle's GPU.
+ *
+ * One handle is one lattice of one model resident on one GPU: the state
+ * (Q bit-planes, ping-pong), the cell-type masks, the output snapshot and the
+ * coarse fields.  Work is queued on the handle's HIP stream; entry points that
+ * return data to the host wait for it.  A handle may be driven by two host
+ * threads at once only as the reference's viewers do (one thread stepping and
+ * snapshotting, another post-processing the snapshot); otherwise a handle is
+ * not re-entrant.
+ *
+ * Reference interface replaced: template<Model> class Lattice, src/lattice.h:32-238,
+ * whose only live implementation is OMP_Lattice, src/omp_lattice.h:27-78.
+ */
+#ifndef LGCA_H_
+#define LGCA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGCA_API __attribute__((visibility("default")))
+
+typedef int lgca_status;
+enum {
+    LGCA_OK = 0,
+    LGCA_ERR_INVALID_ARGUMENT = 1, /* bad size, model, range or pointer     */
+    LGCA_ERR_HIP = 2,              /* a HIP runtime call failed             */
+    LGCA_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed              */
+    LGCA_ERR_INVALID_STATE = 4,    /* state bytes carry bits >= Q, or cell
+                                      types outside 0..2                    */
+    LGCA_ERR_BUDGET = 5            /* request exceeds an HBM budget         */
+};
+
+/* Models, src/lgca_common.h:49-54. */
+enum { LGCA_HPP = 0, LGCA_FHP_I = 1, LGCA_FHP_II = 2, LGCA_FHP_III = 3 };
+/* Cell types, src/lgca_common.h:56-60 (one byte per cell at this ABI). */
+enum { LGCA_FLUID = 0, LGCA_SOLID_NO_SLIP = 1, LGCA_SOLID_SLIP = 2 };
+/* Body-force direction, m_bf_dir of src/lattice.h:68 ('x' or 'y'). */
+enum { LGCA_FORCE_NONE = 0, LGCA_FORCE_X = 1, LGCA_FORCE_Y = 2 };
+
+typedef struct lgca_lattice lgca_lattice;
+
+typedef struct lgca_info {
+    int32_t model;
+    uint32_t num_dir;        /* Q: 4, 6, 7, 7                                   */
+    uint32_t dim_x, dim_y;
+    uint32_t coarse_radius;  /* r                                              */
+    uint32_t coarse_dim_x, coarse_dim_y;
+    uint32_t words_per_row;  /* 32-cell words per row and plane                */
+    uint32_t row_pitch;      /* words between consecutive planes (>= words)   */
+    uint32_t reserved;
+    uint64_t num_cells;
+    uint64_t num_coarse_cells;
+    uint64_t state_bytes;    /* one state buffer: dim_y * Q * row_pitch * 4   */
+    uint64_t device_bytes;   /* everything the handle holds on the GPU        */
+    uint64_t step;           /* absolute step counter (hash input)            */
+    uint64_t seed;
+} lgca_info;
+
+/* Creates a lattice of dim_x x dim_y cells on GPU `device`, all FLUID and
+ * empty.  Replaces the constructors Lattice(...) / OMP_Lattice(...)
+ * (src/lattice.cpp:29-157, src/omp_lattice.cpp:74-88), which derive the size
+ * from (test case, Re, Ma, r); the size is explicit here and the derivation
+ * lives in the C++ wrapper.  FHP models need an even dim_y, as the reference
+ * asserts (src/lattice.cpp:141).  The coarse grid has floor(dim / 2r) cells per
+ * axis, which is what the reference computes when its divisibility asserts
+ * (src/lattice.cpp:152-153) are compiled out; both sizes must be >= 2r. */
+LGCA_API lgca_status lgca_create(lgca_lattice** out, int model, uint32_t dim_x, uint32_t dim_y,
+                        uint32_t coarse_radius, int device);
+/* Frees the handle; replaces ~OMP_Lattice / free_memory (src/omp_lattice.cpp:91-97,473-489). */
+LGCA_API lgca_status lgca_destroy(lgca_lattice* h);
+/* Text of the last failure on h (or of the last failed lgca_create when h is NULL). */
+LGCA_API const char* lgca_last_error(const lgca_lattice* h);
+LGCA_API lgca_status lgca_get_info(const lgca_lattice* h, lgca_info* info);
+
+/* Queue work on an external HIP stream (a hipStream_t; NULL restores the
+ * handle's own stream). */
+LGCA_API lgca_status lgca_set_stream(lgca_lattice* h, void* hip_stream);
+LGCA_API lgca_status lgca_synchronize(lgca_lattice* h);
+
+/* Seed and absolute step of the counter-based hash that stands in for the
+ * reference's rand() (src/lgca_bitset.h:284-288, src/omp_lattice.cpp:84,269). */
+LGCA_API lgca_status lgca_set_seed(lgca_lattice* h, uint64_t seed);
+LGCA_API lgca_status lgca_set_step(lgca_lattice* h, uint64_t step);
+
+/* Cell types, one byte per cell (0 FLUID, 1 NO_SLIP, 2 SLIP), row-major.
+ * Replaces writes to m_cell_type_cpu by apply_cell_type_all / apply_bc_* /
+ * apply_boundary_cell_type_* (src/lattice.cpp:221-395). */
+LGCA_API lgca_status lgca_set_cell_types(lgca_lattice* h, const uint8_t* host_types);
+LGCA_API lgca_status lgca_get_cell_types(lgca_lattice* h, uint8_t* host_types);
+
+/* State, one byte per cell, bit d = particle in direction d (the Bitset
+ * layout, src/lgca_bitset.h:229-231).  Bytes with bits >= Q are rejected.
+ * Replaces copy_data_to_device / copy_data_from_device (src/lattice.h:206-209). */
+LGCA_API lgca_status lgca_set_state_u8(lgca_lattice* h, const uint8_t* host_state);
+LGCA_API lgca_status lgca_get_state_u8(lgca_lattice* h, uint8_t* host_state);
+/* Device-pointer forms of the same (num_cells bytes on the handle's GPU). */
+LGCA_API lgca_status lgca_set_state_u8_device(lgca_lattice* h, const uint8_t* dev_state);
+LGCA_API lgca_status lgca_get_state_u8_device(lgca_lattice* h, uint8_t* dev_state);
+
+/* Fills every FLUID cell, every direction, with probability p_fill from the
+ * hash (stream 2); other cells keep their state.  Replaces init_random
+ * (src/lattice.cpp:198-217), whose threshold is p = 1/Q. */
+LGCA_API lgca_status lgca_init_random(lgca_lattice* h, double p_fill);
+/* The same restricted to cells whose byte in host_region (num_cells bytes) is
+ * non-zero; replaces init_diffusion (src/lattice.cpp:465-496), whose disc the
+ * caller rasterises. */
+LGCA_API lgca_status lgca_init_random_masked(lgca_lattice* h, double p_fill,
+                                             const uint8_t* host_region);
+
+/* Per-cell body force fused into the step: after collision, each FLUID cell in
+ * [x0, x1) x [y0, y1) is forced with probability p (hash stream 1) using the
+ * flip rule of apply_body_force (src/omp_lattice.cpp:293-337).  dir
+ * LGCA_FORCE_NONE disables it. */
+LGCA_API lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint32_t x0, uint32_t x1,
+                             uint32_t y0, uint32_t y1);
+
+/* Advances nsteps collide-and-propagate steps (src/omp_lattice.cpp:100-249),
+ * each fused with the per-cell force when set.  Queued; returns at once. */
+LGCA_API lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps);
+
+/* Count-mode body force: replaces apply_body_force(int forcing)
+ * (src/omp_lattice.cpp:254-346).  Candidate cells come from the hash (stream
+ * 3, keyed by a per-handle call counter) instead of rand() % N; the loop stops
+ * once `forcing` particles are reverted or 2N candidates are spent, and always
+ * makes at least one try.  *reverted receives the count (may be NULL). */
+LGCA_API lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int64_t forcing,
+                                        int64_t* reverted);
+
+/* Copies the live state into the output snapshot that post-processing reads;
+ * replaces copy_data_to_output_buffer (src/lattice.cpp:437-441). */
+LGCA_API lgca_status lgca_snapshot(lgca_lattice* h);
+
+/* Post-processes the snapshot (src/omp_lattice.cpp:349-454): per-cell density
+ * (N floats) and momentum (2N floats, x/y interleaved), coarse density (Nc)
+ * and coarse momentum (2Nc).  Host pointers; any may be NULL to skip it. */
+LGCA_API lgca_status lgca_post_process(lgca_lattice* h, float* cell_density, float* cell_momentum,
+                              float* mean_density, float* mean_momentum);
+/* Coarse fields left on the GPU: device pointers valid until the next call
+ * or lgca_destroy, filled once the handle's stream reaches this point (see
+ * lgca_synchronize).  velocity = mean momentum / mean density (0 where empty). */
+LGCA_API lgca_status lgca_post_process_device(lgca_lattice* h, const float** mean_density,
+                                     const float** mean_momentum, const float** velocity);
+
+/* Particles in the live state; replaces get_n_particles (src/lattice.cpp:180-195). */
+LGCA_API lgca_status lgca_count_particles(lgca_lattice* h, uint64_t* count);
+/* Mean over FLUID cells of momentum / density of the snapshot; replaces
+ * get_mean_velocity (src/omp_lattice.cpp:508-557).  Summed exactly. */
+LGCA_API lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LGCA_H_ */

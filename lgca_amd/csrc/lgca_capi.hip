@@ -1,0 +1,742 @@
+// lgca_capi.hip -- the C ABI of include/lgca.h over the gfx950 kernels.
+//
+// A handle owns, on its GPU: two state buffers (ping-pong bit-planes), the
+// output snapshot, the cell-type masks with a per-row range of non-fluid words,
+// force-decision planes for a window of upcoming steps, post-processing
+// outputs and a small scratch block for counters and validation flags.
+#include "lgca.h"
+
+#include "lgca_dev.hpp"
+#include "lgca_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+using namespace lgca;
+
+struct lgca_lattice {
+    int model = 0;
+    int kind = 0;
+    uint32_t Q = 0;
+    uint32_t dim_x = 0, dim_y = 0, r = 1;
+    int device = 0;
+    Geometry g{};
+    size_t state_words = 0;
+    uint32_t* state[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint32_t* snap = nullptr;
+    uint32_t* mask = nullptr;       // allocated once any non-fluid cell is set
+    int32_t* row_range = nullptr;
+    uint32_t* dec = nullptr;        // force decisions, [k][rows][fwn]
+    size_t dec_cap = 0;             // words
+    uint64_t dec_step0 = 0;
+    uint32_t dec_count = 0;
+    float* cell_density = nullptr;
+    float* cell_momentum = nullptr;
+    float* mean_density = nullptr;
+    float* mean_momentum = nullptr;
+    float* velocity = nullptr;
+    uint8_t* bytes = nullptr;       // N-byte staging for u8 state / types
+    uint32_t* visited = nullptr;    // count-mode force: first-visit bitmap
+    unsigned long long* scratch = nullptr;  // 8 words: counters and flags
+    uint64_t seed = 0, step = 0, force_calls = 0;
+    int force_dir = LGCA_FORCE_NONE;
+    uint64_t force_thr = 0;
+    uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::mutex mu;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int now = -1;
+        if (prev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev) (void)hipSetDevice(prev);
+    }
+};
+
+lgca_status fail(lgca_lattice* h, lgca_status code, const std::string& msg)
+{
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+lgca_status hip_fail(lgca_lattice* h, hipError_t e, const char* what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(h, e == hipErrorOutOfMemory ? LGCA_ERR_OUT_OF_MEMORY : LGCA_ERR_HIP, m);
+}
+
+#define HIPCHK(h, expr)                                             \
+    do {                                                            \
+        hipError_t e_ = (expr);                                     \
+        if (e_ != hipSuccess) return hip_fail((h), e_, #expr);      \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t bytes)
+{
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, bytes ? bytes : 16);
+    *p = (T*)v;
+    return e;
+}
+
+uint64_t threshold(double p)
+{
+    if (!(p > 0.0)) return 0;
+    if (p >= 1.0) return (uint64_t)1 << 32;
+    return (uint64_t)std::floor(p * 4294967296.0);
+}
+
+lgca_status ensure_bytes(lgca_lattice* h)
+{
+    if (!h->bytes) HIPCHK(h, dalloc(&h->bytes, (size_t)h->dim_x * h->dim_y));
+    return LGCA_OK;
+}
+
+lgca_status ensure_mask(lgca_lattice* h)
+{
+    if (!h->mask) {
+        HIPCHK(h, dalloc(&h->mask, (size_t)h->dim_y * 2 * h->g.P * 4));
+        HIPCHK(h, dalloc(&h->row_range, (size_t)h->dim_y * 2 * 4));
+    }
+    return LGCA_OK;
+}
+
+lgca_status read_scratch(lgca_lattice* h, unsigned long long* out, int n)
+{
+    HIPCHK(h, hipMemcpyAsync(out, h->scratch, n * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LGCA_OK;
+}
+
+lgca_status clear_scratch(lgca_lattice* h)
+{
+    HIPCHK(h, launch_fill_u32((uint32_t*)h->scratch, 0u, 16, h->stream));
+    return LGCA_OK;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------- count force
+// apply_body_force(int forcing), src/omp_lattice.cpp:254-346.  One wave walks
+// the candidate sequence 64 tries at a time.  A revisit of a cell can never
+// flip (after a visit each flip rule of the cell is infeasible), so only first
+// visits count: the wave drops candidates that repeat an earlier lane's cell,
+// tests-and-sets a per-call visited bitmap, sums the reverts of its lanes in
+// order and stops at the first try whose running total reaches `forcing`
+// (always after at least one try), exactly as the sequential loop would.
+namespace lgca {
+
+template <int KIND>
+__global__ __launch_bounds__(64) void body_force_count_kernel(uint32_t* planes, const uint32_t* mask,
+                                                              Geometry g, uint32_t key0, uint32_t key1,
+                                                              uint32_t call, int dir, long long forcing,
+                                                              uint32_t* visited, long long* result)
+{
+    constexpr int Q = num_dir_of_kind(KIND);
+    const uint32_t lane = threadIdx.x;
+    const uint64_t n = (uint64_t)g.dim_x * g.dim_y;
+    const uint64_t it_max = 2 * n;
+    long long reverted = 0;
+    for (uint64_t base = 0; base < it_max; base += 64) {
+        const uint64_t i = base + lane;
+        const bool active = i < it_max;
+        // flips of the previous batch (L2 atomics) must be visible to this one
+        __threadfence();
+        const U4 rr = philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), call, 3u, key0, key1);
+        const uint64_t cell = ((((uint64_t)rr.y) << 32) | rr.x) % n;
+        bool dup = false;
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint64_t cj = __shfl(cell, (int)j, 64);
+            if (j < lane && base + j < it_max && cj == cell) dup = true;
+        }
+        bool first = false;
+        const uint32_t y = (uint32_t)(cell / g.dim_x), x = (uint32_t)(cell % g.dim_x);
+        const uint32_t w = x >> 5, bit = 1u << (x & 31u);
+        if (active && !dup) {
+            const uint32_t old = atomicOr(visited + (cell >> 5), 1u << (cell & 31u));
+            first = (old & (1u << (cell & 31u))) == 0;
+        }
+        uint32_t flip[7] = {0, 0, 0, 0, 0, 0, 0};
+        int k = 0;
+        if (first) {
+            bool fluid = true;
+            if (mask)
+                fluid = ((mask[((size_t)y * 2 + 0) * g.P + w] | mask[((size_t)y * 2 + 1) * g.P + w]) & bit) == 0;
+            if (fluid) {
+                uint32_t s[7], o[7];
+#pragma unroll
+                for (int d = 0; d < Q; ++d)
+                    s[d] = o[d] = __hip_atomic_load(planes + ((size_t)y * Q + d) * g.P + w,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit;
+                apply_force<KIND>(o, bit, dir);
+#pragma unroll
+                for (int d = 0; d < Q; ++d) {
+                    flip[d] = s[d] ^ o[d];
+                    k += flip[d] ? 1 : 0;
+                }
+                k /= 2;
+            }
+        }
+        // inclusive prefix of k over lanes
+        int incl = k;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if ((int)lane >= off) incl += t;
+        }
+        const bool hit = active && (reverted + incl >= forcing);
+        const unsigned long long hits = __ballot(hit);
+        const uint32_t limit = hits ? (uint32_t)__ffsll((long long)hits) - 1u : 63u;
+        if (lane <= limit && first && k > 0) {
+#pragma unroll
+            for (int d = 0; d < Q; ++d)
+                if (flip[d]) atomicXor(planes + ((size_t)y * Q + d) * g.P + w, flip[d]);
+        }
+        if (lane > limit && first) atomicAnd(visited + (cell >> 5), ~(1u << (cell & 31u)));
+        reverted += __shfl(incl, (int)limit, 64);
+        if (hits) break;
+    }
+    if (lane == 0) *result = reverted;
+}
+
+}  // namespace lgca
+
+// --------------------------------------------------------------- lifecycle
+
+extern "C" lgca_status lgca_create(lgca_lattice** out, int model, uint32_t dim_x, uint32_t dim_y,
+                                   uint32_t coarse_radius, int device)
+{
+    if (!out) return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, "lgca_create: out is NULL");
+    *out = nullptr;
+    if (model < LGCA_HPP || model > LGCA_FHP_III)
+        return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, "lgca_create: unknown model");
+    if (dim_x == 0 || dim_y == 0 || coarse_radius == 0)
+        return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, "lgca_create: zero size or radius");
+    if (model != LGCA_HPP && (dim_y % 2) != 0)
+        return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT,
+                    "lgca_create: FHP models need an even dim_y (hexagonal row parity)");
+    if (dim_x < 2 * coarse_radius || dim_y < 2 * coarse_radius)
+        return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT,
+                    "lgca_create: dim_x and dim_y must be at least 2 * coarse_radius");
+    if ((uint64_t)dim_x * dim_y >= (1ull << 40) || dim_y > 65535 || dim_x >= (1u << 30))
+        return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT,
+                    "lgca_create: lattice too large (dim_y <= 65535, dim_x < 2^30)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, "lgca_create: no such GPU");
+
+    auto* h = new lgca_lattice();
+    h->model = model;
+    h->kind = model == LGCA_HPP ? kHPP : (model == LGCA_FHP_I ? kFHP1 : kFHP2);
+    h->Q = (uint32_t)num_dir_of_kind(h->kind);
+    h->dim_x = dim_x;
+    h->dim_y = dim_y;
+    h->r = coarse_radius;
+    h->device = device;
+    h->g = make_geometry(dim_x, dim_y, h->Q);
+    h->state_words = (size_t)dim_y * h->Q * h->g.P;
+    DeviceGuard dg(device);
+    auto bail = [&](lgca_status st) {
+        g_create_error = h->err;
+        lgca_destroy(h);
+        return st;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(h, e, "hipStreamCreate"));
+    h->stream = h->own_stream;
+    const size_t sb = h->state_words * 4;
+    if ((e = dalloc(&h->state[0], sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(state)"));
+    if ((e = dalloc(&h->state[1], sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(state)"));
+    if ((e = dalloc(&h->snap, sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(snapshot)"));
+    if ((e = dalloc(&h->scratch, 16 * sizeof(unsigned long long))) != hipSuccess)
+        return bail(hip_fail(h, e, "hipMalloc(scratch)"));
+    const uint64_t nw = (uint64_t)h->state_words;
+    if ((e = launch_fill_u32(h->state[0], 0u, nw, h->stream)) != hipSuccess ||
+        (e = launch_fill_u32(h->state[1], 0u, nw, h->stream)) != hipSuccess ||
+        (e = launch_fill_u32(h->snap, 0u, nw, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return bail(hip_fail(h, e, "initial fill"));
+    *out = h;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_destroy(lgca_lattice* h)
+{
+    if (!h) return LGCA_OK;
+    {
+        DeviceGuard dg(h->device);
+        if (h->own_stream) (void)hipStreamSynchronize(h->own_stream);
+        if (h->stream && h->stream != h->own_stream) (void)hipStreamSynchronize(h->stream);
+        void* ptrs[] = {h->state[0], h->state[1], h->snap, h->mask, h->row_range, h->dec,
+                        h->cell_density, h->cell_momentum, h->mean_density, h->mean_momentum,
+                        h->velocity, h->bytes, h->visited, h->scratch};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    }
+    delete h;
+    return LGCA_OK;
+}
+
+extern "C" const char* lgca_last_error(const lgca_lattice* h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" lgca_status lgca_get_info(const lgca_lattice* h, lgca_info* info)
+{
+    if (!h || !info) return LGCA_ERR_INVALID_ARGUMENT;
+    std::memset(info, 0, sizeof *info);
+    info->model = h->model;
+    info->num_dir = h->Q;
+    info->dim_x = h->dim_x;
+    info->dim_y = h->dim_y;
+    info->coarse_radius = h->r;
+    info->coarse_dim_x = h->dim_x / (2 * h->r);
+    info->coarse_dim_y = h->dim_y / (2 * h->r);
+    info->words_per_row = h->g.W;
+    info->row_pitch = h->g.P;
+    info->num_cells = (uint64_t)h->dim_x * h->dim_y;
+    info->num_coarse_cells = (uint64_t)info->coarse_dim_x * info->coarse_dim_y;
+    info->state_bytes = (uint64_t)h->state_words * 4;
+    uint64_t dev = 3 * info->state_bytes + 16 * 8;
+    if (h->mask) dev += (uint64_t)h->dim_y * 2 * h->g.P * 4 + (uint64_t)h->dim_y * 8;
+    dev += h->dec_cap * 4;
+    if (h->cell_density) dev += info->num_cells * 4;
+    if (h->cell_momentum) dev += info->num_cells * 8;
+    if (h->mean_density) dev += info->num_coarse_cells * 20;
+    if (h->bytes) dev += info->num_cells;
+    if (h->visited) dev += (info->num_cells + 31) / 32 * 4;
+    info->device_bytes = dev;
+    info->step = h->step;
+    info->seed = h->seed;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_set_stream(lgca_lattice* h, void* hip_stream)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    hipStream_t next = hip_stream ? (hipStream_t)hip_stream : h->own_stream;
+    if (next != h->stream) {
+        // order the new stream after everything queued so far
+        hipEvent_t ev;
+        HIPCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(h, hipEventRecord(ev, h->stream));
+        HIPCHK(h, hipStreamWaitEvent(next, ev, 0));
+        HIPCHK(h, hipEventDestroy(ev));
+        h->stream = next;
+    }
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_synchronize(lgca_lattice* h)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    DeviceGuard dg(h->device);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_set_seed(lgca_lattice* h, uint64_t seed)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->seed = seed;
+    h->dec_count = 0;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_set_step(lgca_lattice* h, uint64_t step)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->step = step;
+    h->dec_count = 0;
+    return LGCA_OK;
+}
+
+// ------------------------------------------------------------ cell types
+
+static lgca_status upload_types(lgca_lattice* h, const uint8_t* types, hipMemcpyKind kind)
+{
+    const size_t n = (size_t)h->dim_x * h->dim_y;
+    lgca_status st = ensure_bytes(h);
+    if (st) return st;
+    HIPCHK(h, hipMemcpyAsync(h->bytes, types, n, kind, h->stream));
+    if ((st = ensure_mask(h))) return st;
+    if ((st = clear_scratch(h))) return st;
+    // row ranges start empty: lo[y] = INT_MAX, hi[y] = 0
+    HIPCHK(h, launch_fill_u32((uint32_t*)h->row_range, 0x7FFFFFFFu, h->dim_y, h->stream));
+    HIPCHK(h, launch_fill_u32((uint32_t*)h->row_range + h->dim_y, 0u, h->dim_y, h->stream));
+    HIPCHK(h, launch_types(h->bytes, h->mask, h->row_range, h->g, (uint32_t*)h->scratch, h->stream));
+    unsigned long long flag = 0;
+    if ((st = read_scratch(h, &flag, 1))) return st;
+    if (flag & 1ull) return fail(h, LGCA_ERR_INVALID_STATE, "cell types must be 0, 1 or 2");
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_set_cell_types(lgca_lattice* h, const uint8_t* host_types)
+{
+    if (!h || !host_types) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    return upload_types(h, host_types, hipMemcpyHostToDevice);
+}
+
+extern "C" lgca_status lgca_get_cell_types(lgca_lattice* h, uint8_t* host_types)
+{
+    if (!h || !host_types) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    const size_t n = (size_t)h->dim_x * h->dim_y;
+    lgca_status st = ensure_bytes(h);
+    if (st) return st;
+    HIPCHK(h, launch_types_unpack(h->mask, h->bytes, h->g, h->stream));
+    HIPCHK(h, hipMemcpyAsync(host_types, h->bytes, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LGCA_OK;
+}
+
+// ----------------------------------------------------------------- state
+
+static lgca_status upload_state(lgca_lattice* h, const uint8_t* src, hipMemcpyKind kind)
+{
+    const size_t n = (size_t)h->dim_x * h->dim_y;
+    lgca_status st = ensure_bytes(h);
+    if (st) return st;
+    HIPCHK(h, hipMemcpyAsync(h->bytes, src, n, kind, h->stream));
+    if ((st = clear_scratch(h))) return st;
+    HIPCHK(h, launch_fill_u32(h->state[h->cur], 0u, h->state_words, h->stream));
+    HIPCHK(h, launch_pack(h->bytes, h->state[h->cur], h->g, (uint32_t*)h->scratch, h->stream));
+    unsigned long long flag = 0;
+    if ((st = read_scratch(h, &flag, 1))) return st;
+    if (flag & 1ull) {
+        HIPCHK(h, launch_fill_u32(h->state[h->cur], 0u, h->state_words, h->stream));
+        return fail(h, LGCA_ERR_INVALID_STATE, "state bytes carry bits at or above num_dir");
+    }
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_set_state_u8(lgca_lattice* h, const uint8_t* host_state)
+{
+    if (!h || !host_state) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    return upload_state(h, host_state, hipMemcpyHostToDevice);
+}
+
+extern "C" lgca_status lgca_set_state_u8_device(lgca_lattice* h, const uint8_t* dev_state)
+{
+    if (!h || !dev_state) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    return upload_state(h, dev_state, hipMemcpyDeviceToDevice);
+}
+
+static lgca_status download_state(lgca_lattice* h, uint8_t* dst, hipMemcpyKind kind)
+{
+    const size_t n = (size_t)h->dim_x * h->dim_y;
+    if (kind == hipMemcpyDeviceToDevice) {
+        HIPCHK(h, launch_unpack(h->state[h->cur], dst, h->g, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return LGCA_OK;
+    }
+    lgca_status st = ensure_bytes(h);
+    if (st) return st;
+    HIPCHK(h, launch_unpack(h->state[h->cur], h->bytes, h->g, h->stream));
+    HIPCHK(h, hipMemcpyAsync(dst, h->bytes, n, kind, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_get_state_u8(lgca_lattice* h, uint8_t* host_state)
+{
+    if (!h || !host_state) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    return download_state(h, host_state, hipMemcpyDeviceToHost);
+}
+
+extern "C" lgca_status lgca_get_state_u8_device(lgca_lattice* h, uint8_t* dev_state)
+{
+    if (!h || !dev_state) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    return download_state(h, dev_state, hipMemcpyDeviceToDevice);
+}
+
+extern "C" lgca_status lgca_init_random(lgca_lattice* h, double p_fill)
+{
+    if (!h || !(p_fill >= 0.0) || p_fill > 1.0) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    HIPCHK(h, launch_init_fill(h->state[h->cur], h->mask, nullptr, h->g, (uint32_t)h->seed,
+                               (uint32_t)(h->seed >> 32), threshold(p_fill), h->stream));
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_init_random_masked(lgca_lattice* h, double p_fill,
+                                               const uint8_t* host_region)
+{
+    if (!h || !host_region || !(p_fill >= 0.0) || p_fill > 1.0) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    lgca_status st = ensure_bytes(h);
+    if (st) return st;
+    HIPCHK(h, hipMemcpyAsync(h->bytes, host_region, (size_t)h->dim_x * h->dim_y,
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, launch_init_fill(h->state[h->cur], h->mask, h->bytes, h->g, (uint32_t)h->seed,
+                               (uint32_t)(h->seed >> 32), threshold(p_fill), h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LGCA_OK;
+}
+
+// ----------------------------------------------------------------- step
+
+extern "C" lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint32_t x0, uint32_t x1,
+                                        uint32_t y0, uint32_t y1)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    if (dir != LGCA_FORCE_NONE && dir != LGCA_FORCE_X && dir != LGCA_FORCE_Y)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_forcing: dir must be NONE, X or Y");
+    if (dir != LGCA_FORCE_NONE && !(p >= 0.0 && p <= 1.0))
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_forcing: p must lie in [0, 1]");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->force_dir = dir;
+    h->force_thr = threshold(p);
+    h->fx0 = x0;
+    h->fx1 = x1 < h->dim_x ? x1 : h->dim_x;
+    h->fy0 = y0;
+    h->fy1 = y1 < h->dim_y ? y1 : h->dim_y;
+    if (h->fx0 >= h->fx1 || h->fy0 >= h->fy1 || h->force_thr == 0) h->force_dir = LGCA_FORCE_NONE;
+    h->dec_count = 0;
+    return LGCA_OK;
+}
+
+static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining)
+{
+    const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
+    const uint32_t fwn = fw1 - fw0, rows = h->fy1 - h->fy0;
+    const size_t per_step = (size_t)fwn * rows;
+    // a window of up to 64 steps, capped at 64 MiB of decision words
+    size_t k = (size_t)(64ull << 20) / 4 / per_step;
+    if (k > 64) k = 64;
+    if (k < 1) k = 1;
+    if (k > remaining) k = (size_t)remaining;
+    if (h->dec_cap < k * per_step) {
+        if (h->dec) {
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            HIPCHK(h, hipFree(h->dec));
+        }
+        h->dec = nullptr;
+        h->dec_cap = 0;
+        size_t want = per_step * (k < 64 ? 64 : k);
+        if (want * 4 > (size_t)(64ull << 20) && per_step * 4 <= (size_t)(64ull << 20)) want = k * per_step;
+        HIPCHK(h, dalloc(&h->dec, want * 4));
+        h->dec_cap = want;
+    }
+    ForceDrawArgs fa{};
+    fa.dec = h->dec;
+    fa.nsteps = (uint32_t)k;
+    fa.step0 = (uint32_t)h->step;
+    fa.key0 = (uint32_t)h->seed;
+    fa.key1 = (uint32_t)(h->seed >> 32);
+    fa.thr = h->force_thr;
+    fa.fx0 = h->fx0;
+    fa.fx1 = h->fx1;
+    fa.fy0 = h->fy0;
+    fa.fy1 = h->fy1;
+    fa.fw0 = fw0;
+    fa.fwn = fwn;
+    fa.dim_x = h->dim_x;
+    HIPCHK(h, launch_force_draw(fa, h->stream));
+    h->dec_step0 = h->step;
+    h->dec_count = (uint32_t)k;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    StepArgs a{};
+    a.g = h->g;
+    a.mask = h->mask;
+    a.row_range = h->row_range;
+    a.key0 = (uint32_t)h->seed;
+    a.key1 = (uint32_t)(h->seed >> 32);
+    a.force_dir = h->force_dir;
+    const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
+    for (uint64_t i = 0; i < nsteps; ++i) {
+        a.fdec = nullptr;
+        if (h->force_dir != LGCA_FORCE_NONE) {
+            if (h->dec_count == 0 || h->step < h->dec_step0 || h->step >= h->dec_step0 + h->dec_count) {
+                lgca_status st = prepare_decisions(h, nsteps - i);
+                if (st) return st;
+            }
+            a.fdec = h->dec + (size_t)(h->step - h->dec_step0) * (fw1 - fw0) * (h->fy1 - h->fy0);
+            a.fy0 = h->fy0;
+            a.fy1 = h->fy1;
+            a.fw0 = fw0;
+            a.fwn = fw1 - fw0;
+        }
+        a.src = h->state[h->cur];
+        a.dst = h->state[h->cur ^ 1];
+        a.step = (uint32_t)h->step;
+        HIPCHK(h, launch_step(h->kind, a, h->stream));
+        h->cur ^= 1;
+        ++h->step;
+    }
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int64_t forcing,
+                                                   int64_t* reverted)
+{
+    if (!h || (dir != LGCA_FORCE_X && dir != LGCA_FORCE_Y)) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    const uint64_t n = (uint64_t)h->dim_x * h->dim_y;
+    const uint64_t vw = (n + 31) / 32;
+    if (!h->visited) HIPCHK(h, dalloc(&h->visited, vw * 4));
+    HIPCHK(h, launch_fill_u32(h->visited, 0u, vw, h->stream));
+    lgca_status st = clear_scratch(h);
+    if (st) return st;
+    const uint32_t call = (uint32_t)h->force_calls++;
+    const uint32_t k0 = (uint32_t)h->seed, k1 = (uint32_t)(h->seed >> 32);
+    long long* res = (long long*)h->scratch;
+    switch (h->kind) {
+    case kHPP: hipLaunchKernelGGL(body_force_count_kernel<kHPP>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
+    case kFHP1: hipLaunchKernelGGL(body_force_count_kernel<kFHP1>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
+    default: hipLaunchKernelGGL(body_force_count_kernel<kFHP2>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
+    }
+    HIPCHK(h, hipGetLastError());
+    unsigned long long v = 0;
+    if ((st = read_scratch(h, &v, 1))) return st;
+    if (reverted) *reverted = (int64_t)v;
+    return LGCA_OK;
+}
+
+// ----------------------------------------------------------- post-process
+
+extern "C" lgca_status lgca_snapshot(lgca_lattice* h)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    HIPCHK(h, hipMemcpyAsync(h->snap, h->state[h->cur], h->state_words * 4, hipMemcpyDeviceToDevice,
+                             h->stream));
+    return LGCA_OK;
+}
+
+static lgca_status run_coarse(lgca_lattice* h)
+{
+    const size_t nc = (size_t)(h->dim_x / (2 * h->r)) * (h->dim_y / (2 * h->r));
+    if (!h->mean_density) {
+        HIPCHK(h, dalloc(&h->mean_density, nc * 4));
+        HIPCHK(h, dalloc(&h->mean_momentum, nc * 8));
+        HIPCHK(h, dalloc(&h->velocity, nc * 8));
+    }
+    HIPCHK(h, launch_coarse(h->kind, h->snap, h->g, h->r, h->mean_density, h->mean_momentum,
+                            h->velocity, h->stream));
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_post_process(lgca_lattice* h, float* cell_density, float* cell_momentum,
+                                         float* mean_density, float* mean_momentum)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    const size_t n = (size_t)h->dim_x * h->dim_y;
+    const size_t nc = (size_t)(h->dim_x / (2 * h->r)) * (h->dim_y / (2 * h->r));
+    if (cell_density || cell_momentum) {
+        if (cell_density && !h->cell_density) HIPCHK(h, dalloc(&h->cell_density, n * 4));
+        if (cell_momentum && !h->cell_momentum) HIPCHK(h, dalloc(&h->cell_momentum, n * 8));
+        HIPCHK(h, launch_cell_fields(h->kind, h->snap, h->g, cell_density ? h->cell_density : nullptr,
+                                     cell_momentum ? h->cell_momentum : nullptr, h->stream));
+        if (cell_density)
+            HIPCHK(h, hipMemcpyAsync(cell_density, h->cell_density, n * 4, hipMemcpyDeviceToHost, h->stream));
+        if (cell_momentum)
+            HIPCHK(h, hipMemcpyAsync(cell_momentum, h->cell_momentum, n * 8, hipMemcpyDeviceToHost, h->stream));
+    }
+    if (mean_density || mean_momentum) {
+        lgca_status st = run_coarse(h);
+        if (st) return st;
+        if (mean_density)
+            HIPCHK(h, hipMemcpyAsync(mean_density, h->mean_density, nc * 4, hipMemcpyDeviceToHost, h->stream));
+        if (mean_momentum)
+            HIPCHK(h, hipMemcpyAsync(mean_momentum, h->mean_momentum, nc * 8, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_post_process_device(lgca_lattice* h, const float** mean_density,
+                                                const float** mean_momentum, const float** velocity)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    lgca_status st = run_coarse(h);
+    if (st) return st;
+    if (mean_density) *mean_density = h->mean_density;
+    if (mean_momentum) *mean_momentum = h->mean_momentum;
+    if (velocity) *velocity = h->velocity;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_count_particles(lgca_lattice* h, uint64_t* count)
+{
+    if (!h || !count) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    lgca_status st = clear_scratch(h);
+    if (st) return st;
+    HIPCHK(h, launch_count(h->state[h->cur], h->state_words, h->scratch, h->stream));
+    unsigned long long v = 0;
+    if ((st = read_scratch(h, &v, 1))) return st;
+    *count = v;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2])
+{
+    if (!h || !out) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    lgca_status st = clear_scratch(h);
+    if (st) return st;
+    HIPCHK(h, launch_mean_velocity(h->kind, h->snap, h->mask, h->g, (long long*)h->scratch, h->stream));
+    unsigned long long v[3] = {0, 0, 0};
+    if ((st = read_scratch(h, v, 3))) return st;
+    const double sx = (double)(long long)v[0], sy = (double)(long long)v[1];
+    const double nf = (double)v[2];
+    const double unit_y = h->kind == kHPP ? 1.0 : (double)0.866025388240814208984375f;
+    out[0] = nf > 0 ? (float)(sx / (840.0 * nf)) : 0.0f;
+    out[1] = nf > 0 ? (float)(sy * unit_y / (840.0 * nf)) : 0.0f;
+    return LGCA_OK;
+}

@@ -1,0 +1,173 @@
+// lgca_dev.hpp -- device building blocks of the lattice-gas engine (gfx950).
+//
+// Cells are multi-spin coded: one bit per cell in 32-bit words, one word per
+// direction ("plane") per 32 consecutive cells of a row.  Bit b of word w of a
+// row is cell x = 32*w + b.  Every rule below is pure bitwise logic on such
+// words, so one VALU op advances 32 cells; gfx950 fuses most 3-input
+// expressions into a single v_bitop3_b32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lgca {
+
+// Models (src/lgca_common.h:49-54).  FHP-III collides exactly like FHP-II in the
+// reference (every FHP-III-only term of src/lgca_models.h:816-839 is
+// identically zero), so kernels are instantiated for kinds 0..2 only.
+enum : int { kHPP = 0, kFHP1 = 1, kFHP2 = 2 };
+
+// Four consecutive words of one plane-row (128 cells).  Bitwise operators act
+// element-wise, so every rule below is written once for T = uint32_t (one
+// word) and T = u32x4 (a quad); the compiler emits one VALU op per element.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int num_dir_of_kind(int kind) { return kind == kHPP ? 4 : (kind == kFHP1 ? 6 : 7); }
+
+// --------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, SC'11.  The key schedule is wave-uniform, so the
+// per-round Weyl increments stay in scalar registers.
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// ------------------------------------------------------------------- collision
+// HPP (src/lgca_models.h:134-152): an isolated head-on pair along one axis
+// turns into a pair along the other.
+template <typename T>
+__host__ __device__ __forceinline__ void collide_hpp(T s[4])
+{
+    const T h = (s[0] & s[2] & ~(s[1] | s[3])) | (s[1] & s[3] & ~(s[0] | s[2]));
+    s[0] ^= h; s[1] ^= h; s[2] ^= h; s[3] ^= h;
+}
+
+// FHP-I / FHP-II (src/lgca_models.h:367-395, 568-613), restated by occupancy.
+// s[0..5] moving directions, s[6] rest (FHP-II), p chirality (1 = clockwise).
+//   * exactly two moving particles, head-on {k, k+3}: rotate by +60 degrees
+//     (p = 0) or -60 degrees (p = 1), rest untouched;
+//   * exactly three, {0,2,4} or {1,3,5}: swap, rest untouched;
+//   * FHP-II: rest + exactly one moving particle i -> particles i-1 and i+1;
+//     exactly two moving at i-1 and i+1 without rest -> rest + particle i.
+// The occupancy count comes from a bit-sliced adder tree.
+template <bool WITH_REST, typename T>
+__host__ __device__ __forceinline__ void collide_fhp(T s[7], T p)
+{
+    const T a0 = s[0], a1 = s[1], a2 = s[2], a3 = s[3], a4 = s[4], a5 = s[5];
+    // 6-input population count: count = ones + 2 * twos + 4 * fours
+    const T x1 = a0 ^ a1 ^ a2, c1 = (a0 & a1) | (a2 & (a0 ^ a1));
+    const T x2 = a3 ^ a4 ^ a5, c2 = (a3 & a4) | (a5 & (a3 ^ a4));
+    const T ones = x1 ^ x2, cx = x1 & x2;
+    const T twos = c1 ^ c2 ^ cx, fours = (c1 & c2) | (cx & (c1 ^ c2));
+    const T o2 = ~ones & twos & ~fours;  // exactly two moving
+    const T o3 = ones & twos & ~fours;   // exactly three moving
+    // head-on pairs and symmetric triples
+    const T h0 = o2 & a0 & a3, h1 = o2 & a1 & a4, h2 = o2 & a2 & a5;
+    const T t = o3 & ((a0 & a2 & a4) | (a1 & a3 & a5));
+    // class m = d mod 3 flips when its own pair leaves or a pair rotates in:
+    // p = 0 moves pair k to class k+1, p = 1 to class k+2.
+    const T A0 = t | h0 | (p & h1) | (~p & h2);
+    const T A1 = t | h1 | (p & h2) | (~p & h0);
+    const T A2 = t | h2 | (p & h0) | (~p & h1);
+    if (WITH_REST) {
+        const T r = s[6];
+        const T o1 = ones & ~twos & ~fours;  // exactly one moving
+        const T e = o1 & r;                  // rest + lone particle
+        const T g = o2 & ~r;                 // two moving, no rest
+        // u_i: a rest-rule event centred on direction i (lone particle at i, or
+        // the pair at i-1, i+1); it flips directions i-1, i, i+1 and the rest bit.
+        const T u0 = (e & a0) | (g & a5 & a1);
+        const T u1 = (e & a1) | (g & a0 & a2);
+        const T u2 = (e & a2) | (g & a1 & a3);
+        const T u3 = (e & a3) | (g & a2 & a4);
+        const T u4 = (e & a4) | (g & a3 & a5);
+        const T u5 = (e & a5) | (g & a4 & a0);
+        s[0] = a0 ^ (A0 | u5 | u0 | u1);
+        s[1] = a1 ^ (A1 | u0 | u1 | u2);
+        s[2] = a2 ^ (A2 | u1 | u2 | u3);
+        s[3] = a3 ^ (A0 | u2 | u3 | u4);
+        s[4] = a4 ^ (A1 | u3 | u4 | u5);
+        s[5] = a5 ^ (A2 | u4 | u5 | u0);
+        s[6] = r ^ (u0 | u1 | u2 | u3 | u4 | u5);
+    } else {
+        s[0] = a0 ^ A0; s[1] = a1 ^ A1; s[2] = a2 ^ A2;
+        s[3] = a3 ^ A0; s[4] = a4 ^ A1; s[5] = a5 ^ A2;
+    }
+}
+
+template <int KIND, typename T>
+__host__ __device__ __forceinline__ void collide(T s[7], T p)
+{
+    if (KIND == kHPP) collide_hpp<T>(s);
+    else collide_fhp<KIND == kFHP2, T>(s, p);
+}
+
+// ---------------------------------------------------------- solid-cell rules
+// Direction permutations (src/lgca_models.h:44-48, 239-243, 446-450):
+//   inverse: HPP d+2, FHP d+3; mirror in x: -d; mirror in y: HPP 2-d, FHP 3-d
+//   (mod 4 or 6; the rest direction maps to itself).
+template <int KIND> __host__ __device__ constexpr int inv_dir(int d)
+{ return KIND == kHPP ? ((d + 2) & 3) : (d == 6 ? 6 : (d + 3) % 6); }
+template <int KIND> __host__ __device__ constexpr int mirx_dir(int d)
+{ return KIND == kHPP ? ((4 - d) & 3) : (d == 6 ? 6 : (6 - d) % 6); }
+template <int KIND> __host__ __device__ constexpr int miry_dir(int d)
+{ return KIND == kHPP ? ((6 - d) & 3) : (d == 6 ? 6 : (9 - d) % 6); }
+
+// Applies the destination-cell dispatch of src/omp_lattice.cpp:193-231 to one
+// word: fluid bits keep the collided value c, NO_SLIP bits take the bounce-back
+// of the pulled value in, SLIP bits take the x-mirror on the north/south rows,
+// the y-mirror on the east/west columns (y wins at corners), and pass through
+// elsewhere.  ns, sl: NO_SLIP and SLIP masks; edge: cells on x = 0 or
+// x = dim_x - 1; ns_row: the word lies on row 0 or dim_y - 1.
+template <int KIND, typename T>
+__host__ __device__ __forceinline__ void apply_solids(T c[7], const T in[7], T ns, T sl, T edge,
+                                                      bool ns_row)
+{
+    constexpr int Q = num_dir_of_kind(KIND);
+    const T fl = ~(ns | sl);
+    T o[7];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+        T sp = ns_row ? in[mirx_dir<KIND>(d)] : in[d];
+        sp = (edge & in[miry_dir<KIND>(d)]) | (~edge & sp);
+        o[d] = (fl & c[d]) | (ns & in[inv_dir<KIND>(d)]) | (sl & sp);
+    }
+#pragma unroll
+    for (int d = 0; d < Q; ++d) c[d] = o[d];
+}
+
+// Body-force flip rules of apply_body_force (src/omp_lattice.cpp:293-337) on
+// the cells selected by f (already restricted to FLUID cells).
+template <int KIND, typename T>
+__host__ __device__ __forceinline__ void apply_force(T s[7], T f, int dir)
+{
+    if (KIND == kHPP) {
+        if (dir == 1) { const T m = f & ~s[0] & s[2]; s[0] ^= m; s[2] ^= m; }
+        else          { const T m = f & s[1] & ~s[3]; s[1] ^= m; s[3] ^= m; }
+    } else {
+        if (dir == 1) { const T m = f & ~s[0] & s[3]; s[0] ^= m; s[3] ^= m; }
+        else {
+            const T m1 = f & s[1] & ~s[5];
+            const T m2 = f & s[2] & ~s[4];
+            s[1] ^= m1; s[5] ^= m1;
+            s[2] ^= m2; s[4] ^= m2;
+        }
+    }
+}
+
+}  // namespace lgca

@@ -1,0 +1,76 @@
+// lgca_kernels.hpp -- launch interface of the lattice-gas kernels.
+//
+// HBM layout of one state buffer ("row-interleaved bit-planes"):
+//   word[(y * Q + d) * P + w]   bit b = cell (32*w + b, y), direction d
+// P (row pitch) is the 32-cell word count W rounded up to a multiple of 4 so
+// every thread moves whole 16-byte quads; bits past dim_x are kept zero.
+// Cell-type masks use the same addressing with two planes per row:
+//   mask[(y * 2 + 0) * P + w]  NO_SLIP,   mask[(y * 2 + 1) * P + w]  SLIP.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lgca {
+
+struct Geometry {
+    uint32_t dim_x, dim_y;
+    uint32_t W;    // 32-cell words per row
+    uint32_t P;    // row pitch in words, multiple of 4
+    uint32_t Wq;   // 4-word quads per row (P / 4)
+    uint32_t nb;   // valid bits in word W-1, 1..32
+    uint32_t Q;    // directions
+    uint32_t qb_log;  // log2 of quads per workgroup row segment (<= 6)
+};
+
+Geometry make_geometry(uint32_t dim_x, uint32_t dim_y, uint32_t Q);
+
+struct StepArgs {
+    Geometry g;
+    const uint32_t* src;
+    uint32_t* dst;
+    const uint32_t* mask;       // cell-type masks, or null when all FLUID
+    const int32_t* row_range;   // lo[dim_y] then hi[dim_y]: words [lo, hi) of row y hold non-fluid cells
+    uint32_t key0, key1;        // hash key = seed
+    uint32_t step;              // hash step (absolute step mod 2^32)
+    const uint32_t* fdec;       // force decisions of this step, or null
+    uint32_t fy0, fy1;          // rows with decisions
+    uint32_t fw0, fwn;          // first word and word count with decisions
+    int32_t force_dir;          // 1 x, 2 y
+};
+
+hipError_t launch_step(int kind, const StepArgs& a, hipStream_t s);
+
+struct ForceDrawArgs {
+    uint32_t* dec;              // [steps][fy1 - fy0][fwn]
+    uint32_t nsteps;
+    uint32_t step0;
+    uint32_t key0, key1;
+    uint64_t thr;               // fire when draw < thr (thr <= 2^32)
+    uint32_t fx0, fx1, fy0, fy1, fw0, fwn;
+    uint32_t dim_x;
+};
+hipError_t launch_force_draw(const ForceDrawArgs& a, hipStream_t s);
+
+hipError_t launch_pack(const uint8_t* state, uint32_t* planes, const Geometry& g,
+                       uint32_t* err, hipStream_t s);
+hipError_t launch_unpack(const uint32_t* planes, uint8_t* state, const Geometry& g, hipStream_t s);
+hipError_t launch_types(const uint8_t* types, uint32_t* mask, int32_t* row_range,
+                        const Geometry& g, uint32_t* err, hipStream_t s);
+hipError_t launch_types_unpack(const uint32_t* mask, uint8_t* types, const Geometry& g,
+                               hipStream_t s);
+hipError_t launch_init_fill(uint32_t* planes, const uint32_t* mask, const uint8_t* region,
+                            const Geometry& g, uint32_t key0, uint32_t key1, uint64_t thr,
+                            hipStream_t s);
+hipError_t launch_cell_fields(int kind, const uint32_t* planes, const Geometry& g,
+                              float* density, float* momentum, hipStream_t s);
+hipError_t launch_coarse(int kind, const uint32_t* planes, const Geometry& g, uint32_t r,
+                         float* mean_density, float* mean_momentum, float* velocity,
+                         hipStream_t s);
+hipError_t launch_count(const uint32_t* words, uint64_t nwords, unsigned long long* out,
+                        hipStream_t s);
+hipError_t launch_mean_velocity(int kind, const uint32_t* planes, const uint32_t* mask,
+                                const Geometry& g, long long* acc3, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+}  // namespace lgca

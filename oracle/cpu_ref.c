@@ -259,13 +259,25 @@ void oracle_init_random(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t
      * occupied with probability p; solid cells are left as they are. */
     const int q = oracle_num_dir(model);
     const uint64_t thr = oracle_threshold(p_fill);
-    for (uint32_t y = 0; y < dim_y; ++y) {
+    uint32_t key[2];
+    seed_key(seed, key);
+#pragma omp parallel for schedule(static)
+    for (long long yl = 0; yl < (long long)dim_y; ++yl) {
+        const uint32_t y = (uint32_t)yl;
+        uint32_t draws[7][4];
         for (uint32_t x = 0; x < dim_x; ++x) {
+            if ((x & 3u) == 0 || x == 0) {
+                /* one Philox call yields the draws of 4 consecutive cells */
+                for (int d = 0; d < q; ++d) {
+                    const uint32_t ctr[4] = { x >> 2, y, (uint32_t)d, 2u };
+                    oracle_philox4x32_10(ctr, key, draws[d]);
+                }
+            }
             const size_t cell = (size_t)y * dim_x + x;
             if (types && types[cell] != 0) continue;
             uint8_t s = state[cell];
             for (int d = 0; d < q; ++d) {
-                if ((uint64_t)oracle_fill_draw(seed, x, y, (uint32_t)d) < thr) s |= (uint8_t)(1u << d);
+                if ((uint64_t)draws[d][x & 3u] < thr) s |= (uint8_t)(1u << d);
                 else s &= (uint8_t)~(1u << d);
             }
             state[cell] = s;
@@ -321,14 +333,25 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
         uint32_t fgroup = UINT32_MAX;
         const int row_forced = force_dir != ORACLE_FORCE_NONE && y >= fy0 && y < fy1;
         const int on_ns = (y == 0) || (y == dim_y - 1);
+        /* Source row and x offset of every direction for this row, taken from
+         * oracle_pull_source at x = 1 (the offset is the same for every x). */
+        const uint8_t* src_row[7];
+        int src_dx[7];
+        for (int d = 0; d < q; ++d) {
+            uint32_t sx, sy;
+            oracle_pull_source(model, dim_x > 2 ? dim_x : 3, dim_y, 1, y, d, &sx, &sy);
+            src_row[d] = in + (size_t)sy * dim_x;
+            src_dx[d] = (int)sx - 1;
+        }
         for (uint32_t x = 0; x < dim_x; ++x) {
             const size_t cell = (size_t)y * dim_x + x;
             /* Pull stream (src/omp_lattice.cpp:141-180). */
             uint8_t s = 0;
             for (int d = 0; d < q; ++d) {
-                uint32_t sx, sy;
-                oracle_pull_source(model, dim_x, dim_y, x, y, d, &sx, &sy);
-                s |= (uint8_t)(in[(size_t)sy * dim_x + sx] & (1u << d));
+                long sx = (long)x + src_dx[d];
+                if (sx < 0) sx += dim_x;
+                else if (sx >= (long)dim_x) sx -= dim_x;
+                s |= (uint8_t)(src_row[d][sx] & (1u << d));
             }
             /* Destination-cell dispatch (src/omp_lattice.cpp:193-231). */
             const uint8_t type = types ? types[cell] : 0;

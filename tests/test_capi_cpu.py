@@ -1,0 +1,76 @@
+"""CPU-side checks of the engine's boundary: the C-ABI library loads and exports
+every entry point include/lgca.h declares, the ctypes signatures cover them,
+and the host-side mirror derives the reference's lattice sizes.  No compute
+call is made here (there may be no GPU)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "lgca.h")
+
+
+def declared_entry_points():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"LGCA_API\s+[\w\s\*]+?\b(lgca_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_expected_surface():
+    names = declared_entry_points()
+    for must in ("lgca_create", "lgca_destroy", "lgca_step", "lgca_post_process",
+                 "lgca_set_cell_types", "lgca_set_state_u8", "lgca_get_state_u8",
+                 "lgca_apply_body_force_count", "lgca_count_particles", "lgca_mean_velocity"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from lgca_amd import _capi
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("liblgca_hip.so not built")
+    L = _capi.lib()
+    for name in declared_entry_points():
+        assert hasattr(L, name), name
+
+
+def test_ctypes_signatures_cover_the_header():
+    from lgca_amd import _capi
+    names = set(declared_entry_points())
+    bound = set(_capi.SIGNATURES) | {"lgca_last_error"}
+    assert names == bound
+
+
+def test_library_refuses_to_load_when_missing(tmp_path, monkeypatch):
+    from lgca_amd import _capi
+    monkeypatch.setattr(_capi, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(_capi, "_lib", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _capi.lib()
+
+
+@pytest.mark.parametrize("case,model,Re,Ma,r,dims", [
+    # the reference viewers' set-ups (apps/*/ *_viewer.h) and the stale
+    # periodic app's default
+    ("pipe", "FHP_III", 80.0, 0.3, 10, (1480, 740)),
+    ("karman", "FHP_III", 80.0, 0.3, 20, (4400, 2200)),
+    ("diffusion", "FHP_III", 200.0, 0.2, 1, (202, 202)),
+    ("collision", "FHP_III", 80.0, 0.2, 1, (21, 10)),
+    ("periodic", "HPP", 240.0, 0.3, 8, (256, 256)),
+])
+def test_reference_size_derivation(case, model, Re, Ma, r, dims):
+    from lgca_amd.lattice import derive_parameters
+    p = derive_parameters(model, case, Re, Ma, r)
+    assert (p["dim_x"], p["dim_y"]) == dims
+    assert p["bf_dir"] == ("x" if case in ("pipe", "karman") else None)
+
+
+def test_disc_matches_float_rounded_distance():
+    from lgca_amd.lattice import disc
+    m = disc(64, 32, 20, 16, 11.0).reshape(32, 64)
+    ys, xs = np.nonzero(m)
+    d = np.sqrt((xs - 20.0) ** 2 + (ys - 16.0) ** 2)
+    assert (d < 5.5).all()
+    assert m.sum() == sum(1 for y in range(32) for x in range(64)
+                          if np.float32(np.sqrt((x - 20) ** 2 + (y - 16) ** 2)) < 5.5)
