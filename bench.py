@@ -94,6 +94,9 @@ def main() -> int:
     ap.add_argument("--workload", default="karman", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--steps-per-pass", type=int, default=None,
+                    help="steps fused per HBM pass (1, 2, 4, 8; 0 = one-step quad kernel)")
+    ap.add_argument("--tile-rows", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -103,6 +106,8 @@ def main() -> int:
     from lgca_amd import hiprt
     w = WORKLOADS[args.workload]
     lat = build_lattice(w)
+    if args.steps_per_pass is not None:
+        lat.set_tuning(args.steps_per_pass, args.tile_rows)
     # the step kernels run on this stream, so events recorded on it bracket them
     stream = hiprt.Stream()
     lat.set_stream(stream.ptr.value)

@@ -129,6 +129,12 @@ LGCA_API lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint32
  * each fused with the per-cell force when set.  Queued; returns at once. */
 LGCA_API lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps);
 
+/* Launch shape of lgca_step: steps fused per pass over HBM (1, 2, 4 or 8; 0
+ * selects the one-step kernel that keeps a 128-cell quad per thread) and rows
+ * per tile (0 = chosen from the lattice size).  Results do not depend on it.
+ * No counterpart in the reference; used by the autotuner and benchmarks. */
+LGCA_API lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows);
+
 /* Count-mode body force: replaces apply_body_force(int forcing)
  * (src/omp_lattice.cpp:254-346).  Candidate cells come from the hash (stream
  * 3, keyed by a per-handle call counter) instead of rand() % N; the loop stops

@@ -46,6 +46,8 @@ struct lgca_lattice {
     int force_dir = LGCA_FORCE_NONE;
     uint64_t force_thr = 0;
     uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
+    int tile_k = 4;                 // steps per pass of the tiled kernel (0: quad kernel)
+    uint32_t tile_th = 0;           // rows per tile, 0 = automatic
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
@@ -535,16 +537,18 @@ extern "C" lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint
     return LGCA_OK;
 }
 
-static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining)
+// Force decisions are drawn for a window of upcoming steps at once; a window
+// always starts at the current step and covers at least `need` steps.
+static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32_t need)
 {
     const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
     const uint32_t fwn = fw1 - fw0, rows = h->fy1 - h->fy0;
     const size_t per_step = (size_t)fwn * rows;
-    // a window of up to 64 steps, capped at 64 MiB of decision words
+    // up to 64 steps, capped at 64 MiB of decision words, never fewer than `need`
     size_t k = (size_t)(64ull << 20) / 4 / per_step;
     if (k > 64) k = 64;
-    if (k < 1) k = 1;
     if (k > remaining) k = (size_t)remaining;
+    if (k < need) k = need;
     if (h->dec_cap < k * per_step) {
         if (h->dec) {
             HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -552,10 +556,8 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining)
         }
         h->dec = nullptr;
         h->dec_cap = 0;
-        size_t want = per_step * (k < 64 ? 64 : k);
-        if (want * 4 > (size_t)(64ull << 20) && per_step * 4 <= (size_t)(64ull << 20)) want = k * per_step;
-        HIPCHK(h, dalloc(&h->dec, want * 4));
-        h->dec_cap = want;
+        HIPCHK(h, dalloc(&h->dec, k * per_step * 4));
+        h->dec_cap = k * per_step;
     }
     ForceDrawArgs fa{};
     fa.dec = h->dec;
@@ -577,38 +579,98 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining)
     return LGCA_OK;
 }
 
+// Rows per tile: tall enough to amortise the 2K halo rows, short enough to
+// give the GPU a few thousand waves.
+static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
+{
+    if (h->tile_th) return h->tile_th;
+    const uint64_t strips = (h->g.W + 61) / 62;
+    uint64_t th = (uint64_t)h->dim_y * strips / 4096;
+    const uint64_t lo = k >= 4 ? 8u * (uint64_t)k : 16u;
+    if (th < lo) th = lo;
+    if (th > 256) th = 256;
+    if (th > h->dim_y) th = h->dim_y;
+    return (uint32_t)th;
+}
+
+extern "C" lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    if (steps_per_pass != 0 && steps_per_pass != 1 && steps_per_pass != 2 && steps_per_pass != 4 &&
+        steps_per_pass != 8)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_tuning: steps_per_pass must be 0, 1, 2, 4 or 8");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->tile_k = steps_per_pass;
+    h->tile_th = tile_rows;
+    return LGCA_OK;
+}
+
 extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
 {
     if (!h) return LGCA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
-    StepArgs a{};
-    a.g = h->g;
-    a.mask = h->mask;
-    a.row_range = h->row_range;
-    a.key0 = (uint32_t)h->seed;
-    a.key1 = (uint32_t)(h->seed >> 32);
-    a.force_dir = h->force_dir;
+    const bool forcing = h->force_dir != LGCA_FORCE_NONE;
     const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
-    for (uint64_t i = 0; i < nsteps; ++i) {
-        a.fdec = nullptr;
-        if (h->force_dir != LGCA_FORCE_NONE) {
-            if (h->dec_count == 0 || h->step < h->dec_step0 || h->step >= h->dec_step0 + h->dec_count) {
-                lgca_status st = prepare_decisions(h, nsteps - i);
+    const size_t per_step = (size_t)(fw1 - fw0) * (h->fy1 - h->fy0);
+    uint64_t left = nsteps;
+    while (left > 0) {
+        // largest block the hash alignment allows: K >= 4 needs step % 4 == 0,
+        // K == 2 an even step
+        int k = 1;
+        if (h->tile_k >= 2 && left >= 2 && (h->step & 1) == 0) k = 2;
+        if (h->tile_k >= 4 && left >= 4 && (h->step & 3) == 0) k = 4;
+        if (h->tile_k >= 8 && left >= 8 && (h->step & 3) == 0) k = 8;
+        if (h->tile_k == 0) k = 1;
+        const uint32_t* fdec = nullptr;
+        if (forcing) {
+            if (h->dec_count == 0 || h->step < h->dec_step0 ||
+                h->step + k > h->dec_step0 + h->dec_count) {
+                lgca_status st = prepare_decisions(h, left, (uint32_t)k);
                 if (st) return st;
             }
-            a.fdec = h->dec + (size_t)(h->step - h->dec_step0) * (fw1 - fw0) * (h->fy1 - h->fy0);
+            fdec = h->dec + (size_t)(h->step - h->dec_step0) * per_step;
+        }
+        if (h->tile_k == 0) {
+            StepArgs a{};
+            a.g = h->g;
+            a.mask = h->mask;
+            a.row_range = h->row_range;
+            a.key0 = (uint32_t)h->seed;
+            a.key1 = (uint32_t)(h->seed >> 32);
+            a.force_dir = h->force_dir;
+            a.fdec = fdec;
             a.fy0 = h->fy0;
             a.fy1 = h->fy1;
             a.fw0 = fw0;
             a.fwn = fw1 - fw0;
+            a.src = h->state[h->cur];
+            a.dst = h->state[h->cur ^ 1];
+            a.step = (uint32_t)h->step;
+            HIPCHK(h, launch_step(h->kind, a, h->stream));
+        } else {
+            TileArgs a{};
+            a.g = h->g;
+            a.src = h->state[h->cur];
+            a.dst = h->state[h->cur ^ 1];
+            a.mask = h->mask;
+            a.row_range = h->row_range;
+            a.key0 = (uint32_t)h->seed;
+            a.key1 = (uint32_t)(h->seed >> 32);
+            a.t0 = (uint32_t)h->step;
+            a.th = auto_tile_rows(h, k);
+            a.fdec = fdec;
+            a.fdec_stride = per_step;
+            a.fy0 = h->fy0;
+            a.fy1 = h->fy1;
+            a.fw0 = fw0;
+            a.fwn = fw1 - fw0;
+            a.force_dir = h->force_dir;
+            HIPCHK(h, launch_tile(h->kind, k, a, h->stream));
         }
-        a.src = h->state[h->cur];
-        a.dst = h->state[h->cur ^ 1];
-        a.step = (uint32_t)h->step;
-        HIPCHK(h, launch_step(h->kind, a, h->stream));
         h->cur ^= 1;
-        ++h->step;
+        h->step += (uint64_t)k;
+        left -= (uint64_t)k;
     }
     return LGCA_OK;
 }

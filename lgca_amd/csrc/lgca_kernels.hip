@@ -151,11 +151,17 @@ __global__ __launch_bounds__(256) void step_kernel(StepArgs a)
 #pragma unroll
     for (int d = 0; d < Q; ++d) s[d] = in[d];
 
-    // Chirality words of this quad: philox({q, y, step, 0})[k] for word 4q+k.
+    // Chirality word of word w: philox({w, y, step >> 2, 0})[step & 3].
     u32x4 p = {0, 0, 0, 0};
     if (KIND != kHPP) {
-        const U4 ch = philox4x32_10(q, y, a.step, 0u, a.key0, a.key1);
-        p = u32x4{ch.x, ch.y, ch.z, ch.w};
+        const uint32_t sel = a.step & 3u;
+        uint32_t pw[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const U4 ch = philox4x32_10(4 * q + k, y, a.step >> 2, 0u, a.key0, a.key1);
+            pw[k] = sel == 0 ? ch.x : (sel == 1 ? ch.y : (sel == 2 ? ch.z : ch.w));
+        }
+        p = u32x4{pw[0], pw[1], pw[2], pw[3]};
     }
     collide<KIND, u32x4>(s, p);
 
@@ -212,12 +218,14 @@ hipError_t launch_step(int kind, const StepArgs& a, hipStream_t s)
 // Decision word (step, y, w): bit b set when cell x = 32w + b lies in
 // [fx0, fx1) and philox({x>>2, y, step, 1})[x & 3] < thr.
 
-__global__ __launch_bounds__(64) void force_draw_kernel(ForceDrawArgs a)
+__global__ __launch_bounds__(256) void force_draw_kernel(ForceDrawArgs a)
 {
-    const uint32_t ww = blockIdx.x * 64 + threadIdx.x;
-    const uint32_t yy = blockIdx.y;
-    const uint32_t k = blockIdx.z;
-    if (ww >= a.fwn) return;
+    const uint32_t rows = a.fy1 - a.fy0;
+    const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (uint64_t)a.nsteps * rows * a.fwn) return;
+    const uint32_t ww = (uint32_t)(idx % a.fwn);
+    const uint32_t yy = (uint32_t)((idx / a.fwn) % rows);
+    const uint32_t k = (uint32_t)(idx / ((uint64_t)a.fwn * rows));
     const uint32_t w = a.fw0 + ww;
     const uint32_t y = a.fy0 + yy;
     const uint32_t step = a.step0 + k;
@@ -239,8 +247,8 @@ __global__ __launch_bounds__(64) void force_draw_kernel(ForceDrawArgs a)
 
 hipError_t launch_force_draw(const ForceDrawArgs& a, hipStream_t s)
 {
-    const dim3 grid(div_up(a.fwn, 64), a.fy1 - a.fy0, a.nsteps);
-    hipLaunchKernelGGL(force_draw_kernel, grid, dim3(64), 0, s, a);
+    const uint64_t n = (uint64_t)a.nsteps * (a.fy1 - a.fy0) * a.fwn;
+    hipLaunchKernelGGL(force_draw_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

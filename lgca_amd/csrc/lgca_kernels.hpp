@@ -41,6 +41,24 @@ struct StepArgs {
 
 hipError_t launch_step(int kind, const StepArgs& a, hipStream_t s);
 
+// Temporally blocked step: K steps per pass, tiles of `th` rows.
+struct TileArgs {
+    Geometry g;
+    const uint32_t* src;
+    uint32_t* dst;
+    const uint32_t* mask;
+    const int32_t* row_range;
+    uint32_t key0, key1;
+    uint32_t t0;                // first step of the block (mod 2^32); K >= 4 needs t0 % 4 == 0,
+                                // K == 2 needs t0 even
+    uint32_t th;                // rows per tile
+    const uint32_t* fdec;       // decisions of step t0, or null
+    uint64_t fdec_stride;       // words between consecutive steps' decisions
+    uint32_t fy0, fy1, fw0, fwn;
+    int32_t force_dir;
+};
+hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s);
+
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]
     uint32_t nsteps;

@@ -171,6 +171,11 @@ class Lattice:
         y1 = self.m_dim_y if y1 is None else y1
         self._check(lib().lgca_set_forcing(self._h, d, float(p), x0, x1, y0, y1))
 
+    def set_tuning(self, steps_per_pass: int, tile_rows: int = 0) -> None:
+        """Launch shape of step(): steps fused per HBM pass (1, 2, 4, 8; 0 = the
+        one-step quad kernel) and rows per tile (0 = automatic)."""
+        self._check(lib().lgca_set_tuning(self._h, int(steps_per_pass), int(tile_rows)))
+
     def step(self, nsteps: int = 1) -> None:
         self.copy_data_to_device()
         self._check(lib().lgca_step(self._h, int(nsteps)))

@@ -75,10 +75,10 @@ uint64_t oracle_threshold(double p)
 uint32_t oracle_chirality_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w)
 {
     uint32_t key[2], out[4];
-    const uint32_t ctr[4] = { w >> 2, y, (uint32_t)step, 0u };
+    const uint32_t ctr[4] = { w, y, (uint32_t)(step >> 2), 0u };
     seed_key(seed, key);
     oracle_philox4x32_10(ctr, key, out);
-    return out[w & 3];
+    return out[step & 3];
 }
 
 uint32_t oracle_force_draw(uint64_t seed, uint64_t step, uint32_t x, uint32_t y)
@@ -328,7 +328,7 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
     seed_key(seed, key);
     for (uint32_t y = y_begin; y < y_end; ++y) {
         uint32_t chir[4] = { 0, 0, 0, 0 };
-        uint32_t chir_group = UINT32_MAX;
+        uint32_t chir_word = UINT32_MAX;
         uint32_t fdraw[4] = { 0, 0, 0, 0 };
         uint32_t fgroup = UINT32_MAX;
         const int row_forced = force_dir != ORACLE_FORCE_NONE && y >= fy0 && y < fy1;
@@ -360,12 +360,12 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
                 int p = 0;
                 if (model != 0) {
                     const uint32_t w = x >> 5;
-                    if ((w >> 2) != chir_group) {
-                        const uint32_t ctr[4] = { w >> 2, y, (uint32_t)step, 0u };
+                    if (w != chir_word) {
+                        const uint32_t ctr[4] = { w, y, (uint32_t)(step >> 2), 0u };
                         oracle_philox4x32_10(ctr, key, chir);
-                        chir_group = w >> 2;
+                        chir_word = w;
                     }
-                    p = (int)((chir[w & 3] >> (x & 31)) & 1u);
+                    p = (int)((chir[step & 3] >> (x & 31)) & 1u);
                 }
                 o = lut->lut[p][s];
                 if (row_forced && x >= fx0 && x < fx1) {

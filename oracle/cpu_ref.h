@@ -14,8 +14,9 @@
  * Random choices (collision chirality, per-cell forcing, initial fill) come
  * from Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
  * as easy as 1, 2, 3", SC'11) keyed by the 64-bit lattice seed:
- *   chirality word (y, w)   = philox(ctr = {w>>2, y, step, 0})[w & 3]
+ *   chirality word (y, w)   = philox(ctr = {w, y, step>>2, 0})[step & 3]
  *                             bit (x & 31) of word x>>5 is cell x's chirality
+ *                             (one call serves a word for four steps)
  *   forcing draw (x, y)     = philox(ctr = {x>>2, y, step, 1})[x & 3]
  *   initial fill (x, y, d)  = philox(ctr = {x>>2, y, d,    2})[x & 3]
  * with key = {seed & 0xffffffff, seed >> 32} and step taken mod 2^32.  A draw

@@ -55,8 +55,10 @@ def random_state(rng, model, n):
 
 
 def run_both(Lattice, model, dx, dy, types, state, seed, step0, nsteps, force=None, r=1,
-             chunks=1):
+             chunks=1, tuning=None):
     lat = Lattice(model, dim_x=dx, dim_y=dy, coarse_graining_radius=r, seed=seed)
+    if tuning is not None:
+        lat.set_tuning(*tuning)
     if types is not None:
         lat.set_cell_types(types)
     lat.set_state_u8(state)
@@ -91,18 +93,36 @@ STEP_CASES = [
 ]
 
 
+TUNINGS = [(0, 0), (1, 0), (2, 0), (4, 0), (8, 0), (4, 5), (8, 3)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"k{t[0]}th{t[1]}")
 @pytest.mark.parametrize("case", STEP_CASES, ids=lambda c: f"m{c[0]}_{c[1]}x{c[2]}_{c[3]}")
-def test_step_bit_exact(Lattice, case):
+def test_step_bit_exact(Lattice, case, tuning):
     model, dx, dy, geom, force = case
     rng = np.random.default_rng(dx * 131 + dy)
     types = make_types(rng, dx, dy, geom)
     state = random_state(rng, model, dx * dy)
-    say(f"[step] model={model} {dx}x{dy} {geom} force={force}")
+    say(f"[step] model={model} {dx}x{dy} {geom} force={force} tuning={tuning}")
     lat, got, want = run_both(Lattice, model, dx, dy, types, state, seed=dx + 7 * dy,
-                              step0=11, nsteps=23, force=force, chunks=3)
+                              step0=11, nsteps=23, force=force, chunks=3, tuning=tuning)
     np.testing.assert_array_equal(got, want)
     assert lat.steps == 11 + 23
     lat.close()
+
+
+@pytest.mark.parametrize("dx", [62 * 32, 62 * 32 + 1, 63 * 32 - 3, 124 * 32, 4 * 62 * 32 + 77])
+@pytest.mark.parametrize("k", [1, 4, 8])
+def test_tile_strip_boundaries(Lattice, dx, k):
+    """Widths around the 62-word stride of the tiled kernel's waves."""
+    dy = 24
+    rng = np.random.default_rng(dx + k)
+    types = make_types(rng, dx, dy, "mixed")
+    state = random_state(rng, 2, dx * dy)
+    say(f"[tile] {dx}x{dy} k={k}")
+    _, got, want = run_both(Lattice, 2, dx, dy, types, state, seed=5, step0=4, nsteps=2 * k + 1,
+                            force=dict(dir="y", p=0.2, x0=dx // 3, x1=dx - 40), tuning=(k, 8))
+    np.testing.assert_array_equal(got, want)
 
 
 def test_single_steps_match_batched_steps(Lattice):

@@ -61,8 +61,9 @@ def test_hash_streams_are_the_stated_philox_outputs(orc):
     L = orc.lib()
     seed = 0x0123456789ABCDEF
     key = [seed & 0xFFFFFFFF, seed >> 32]
-    for (y, w, step) in [(0, 0, 0), (5, 7, 3), (8191, 1023, 4999)]:
-        assert L.oracle_chirality_word(seed, step, y, w) == orc.philox([w >> 2, y, step, 0], key)[w & 3]
+    for (y, w, step) in [(0, 0, 0), (5, 7, 3), (8191, 1023, 4999), (3, 2, 2**33 + 6)]:
+        assert L.oracle_chirality_word(seed, step, y, w) == \
+            orc.philox([w, y, (step >> 2) & 0xFFFFFFFF, 0], key)[step & 3]
     for (x, y, step) in [(0, 0, 0), (63, 17, 12), (32767, 8191, 4999)]:
         assert L.oracle_force_draw(seed, step, x, y) == orc.philox([x >> 2, y, step, 1], key)[x & 3]
     for (x, y, d) in [(0, 0, 0), (9, 3, 6)]:
