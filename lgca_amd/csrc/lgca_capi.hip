@@ -584,7 +584,7 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32
 static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
 {
     if (h->tile_th) return h->tile_th;
-    const uint64_t strips = (h->g.W + 61) / 62;
+    const uint64_t strips = (h->g.W + 63 - 2 * tile_halo(h->g, k)) / (64 - 2 * tile_halo(h->g, k));
     uint64_t th = (uint64_t)h->dim_y * strips / 4096;
     const uint64_t lo = k >= 4 ? 8u * (uint64_t)k : 16u;
     if (th < lo) th = lo;
@@ -659,6 +659,7 @@ extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
             a.key1 = (uint32_t)(h->seed >> 32);
             a.t0 = (uint32_t)h->step;
             a.th = auto_tile_rows(h, k);
+            a.halo = tile_halo(h->g, k);
             a.fdec = fdec;
             a.fdec_stride = per_step;
             a.fy0 = h->fy0;

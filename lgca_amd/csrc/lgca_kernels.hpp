@@ -52,11 +52,13 @@ struct TileArgs {
     uint32_t t0;                // first step of the block (mod 2^32); K >= 4 needs t0 % 4 == 0,
                                 // K == 2 needs t0 even
     uint32_t th;                // rows per tile
+    uint32_t halo;              // lanes per wave edge that are computed but not stored
     const uint32_t* fdec;       // decisions of step t0, or null
     uint64_t fdec_stride;       // words between consecutive steps' decisions
     uint32_t fy0, fy1, fw0, fwn;
     int32_t force_dir;
 };
+uint32_t tile_halo(const Geometry& g, int k);  // smallest valid TileArgs::halo
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s);
 
 struct ForceDrawArgs {

@@ -10,9 +10,14 @@
 //
 // x-halo: a shifted plane needs one bit of the neighbouring word, which the
 // neighbouring lane holds (DPP wave_shr / wave_shl).  Lanes 0 and 63 have no
-// outer neighbour; their error grows one cell per level from the wave edge,
-// so after K <= 31 levels lanes 1..62 are exact.  Waves therefore overlap by
-// one word on each side and store only lanes 1..62.
+// outer neighbour, so the cells next to the wave edges go wrong, and the wrong
+// zone grows by one cell per level.  Read along the lanes, the cells form the
+// row in order (a ragged last word contributes only its nb valid cells), so
+// after K levels the first and last K cells of the wave are suspect.  The
+// host picks a halo of h lanes per side holding at least K cells (h = 1 for
+// full words, 2 when an edge lane may hold a short last word, ceil(K / nb)
+// when the row is a single word); waves advance by 64 - 2h words and store
+// lanes h..63-h.
 // y-halo: the tile reads K extra rows above and below and discards them.
 //
 // Chirality: philox({w, y, t >> 2, 0})[t & 3] serves a word for four steps, so
@@ -24,8 +29,6 @@
 namespace lgca {
 
 namespace {
-
-constexpr int kLanesValid = 62;  // lanes 1..62 of a wave are stored
 
 __device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane i gets lane i-1
 {
@@ -69,19 +72,20 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
     const Geometry& g = a.g;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t strip = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t nstrips = (g.W + kLanesValid - 1) / kLanesValid;
+    const uint32_t stride = 64u - 2u * a.halo;
+    const uint32_t nstrips = (g.W + stride - 1) / stride;
     if (strip >= nstrips) return;  // whole wave
     const uint32_t y0 = blockIdx.y * a.th;
     const uint32_t rows = min(a.th, g.dim_y - y0);
 
-    const int wabs = (int)(strip * kLanesValid) - 1 + (int)lane;
+    const int wabs = (int)(strip * stride) - (int)a.halo + (int)lane;
     LaneCtx c;
     c.w = (uint32_t)((wabs + (int)g.W) % (int)g.W);
     c.nb = g.nb;
     c.first_shl = (c.w == 0) ? 32u - g.nb : 0u;
     c.last = c.w + 1 == g.W;
     c.vmask = (c.last && g.nb < 32) ? ((1u << g.nb) - 1u) : 0xFFFFFFFFu;
-    const bool writer = lane >= 1 && lane <= (uint32_t)kLanesValid && wabs < (int)g.W;
+    const bool writer = lane >= a.halo && lane < 64u - a.halo && wabs < (int)g.W;
 
     // word range of this wave, for the uniform mask / force tests
     const uint32_t* src = a.src;
@@ -263,9 +267,16 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
     }
 }
 
+uint32_t tile_halo(const Geometry& g, int k)
+{
+    if (g.W == 1) return (uint32_t)((k + (int)g.nb - 1) / (int)g.nb);
+    return (int)g.nb < k ? 2u : 1u;
+}
+
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s)
 {
-    const uint32_t nstrips = (a.g.W + kLanesValid - 1) / kLanesValid;
+    if (a.halo < tile_halo(a.g, k) || a.halo > 16) return hipErrorInvalidValue;
+    const uint32_t nstrips = (a.g.W + (64u - 2u * a.halo) - 1) / (64u - 2u * a.halo);
     const dim3 grid((nstrips + 3) / 4, (a.g.dim_y + a.th - 1) / a.th);
     const bool ragged = a.g.nb != 32;
 #define LGCA_TILE_CASE(KD, KK)                                                                 \
