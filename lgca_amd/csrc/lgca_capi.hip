@@ -583,13 +583,15 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32
 // give the GPU a few thousand waves.
 static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
 {
-    if (h->tile_th) return h->tile_th;
+    if (h->tile_th) return h->tile_th + (h->tile_th & 1u);
     const uint64_t strips = (h->g.W + 63 - 2 * tile_halo(h->g, k)) / (64 - 2 * tile_halo(h->g, k));
-    uint64_t th = (uint64_t)h->dim_y * strips / 4096;
+    // about two rounds of resident waves (four per SIMD on 256 CUs)
+    uint64_t th = (uint64_t)h->dim_y * strips / 8192;
     const uint64_t lo = k >= 4 ? 8u * (uint64_t)k : 16u;
     if (th < lo) th = lo;
     if (th > 256) th = 256;
     if (th > h->dim_y) th = h->dim_y;
+    th += th & 1u;  // tiles start on even rows
     return (uint32_t)th;
 }
 

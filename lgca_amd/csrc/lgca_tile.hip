@@ -26,6 +26,8 @@
 #include "lgca_dev.hpp"
 #include "lgca_kernels.hpp"
 
+#include <type_traits>
+
 namespace lgca {
 
 namespace {
@@ -65,8 +67,14 @@ __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 
 }  // namespace
 
+typedef const __attribute__((address_space(4))) int32_t* const_i32_ptr;
+
+// K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
+template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
+
 template <int KIND, int K, bool RAGGED>
-__global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+void tile_kernel(TileArgs a)
 {
     constexpr int Q = num_dir_of_kind(KIND);
     const Geometry& g = a.g;
@@ -75,6 +83,8 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
     const uint32_t stride = 64u - 2u * a.halo;
     const uint32_t nstrips = (g.W + stride - 1) / stride;
     if (strip >= nstrips) return;  // whole wave
+    // tiles start on even rows (a.th is even), so the parity of the row a
+    // level works on follows from the iteration index: see `odd_row` below
     const uint32_t y0 = blockIdx.y * a.th;
     const uint32_t rows = min(a.th, g.dim_y - y0);
 
@@ -86,25 +96,68 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
     c.last = c.w + 1 == g.W;
     c.vmask = (c.last && g.nb < 32) ? ((1u << g.nb) - 1u) : 0xFFFFFFFFu;
     const bool writer = lane >= a.halo && lane < 64u - a.halo && wabs < (int)g.W;
+    uint32_t edge = 0;  // cells of this lane on the east / west columns
+    if (c.w == 0) edge |= 1u;
+    if (c.last) edge |= 1u << (g.nb - 1u);
 
-    // word range of this wave, for the uniform mask / force tests
     const uint32_t* src = a.src;
     uint32_t* dst = a.dst;
     const uint32_t n_iter = rows + 2 * K;
+    const int dy = (int)g.dim_y;
     auto wrap = [&](int r) -> uint32_t {
-        r %= (int)g.dim_y;
-        return (uint32_t)(r < 0 ? r + (int)g.dim_y : r);
+        if (r < 0) {
+            r += dy;
+            if (r < 0) r = (r % dy + dy) % dy;
+        } else if (r >= dy) {
+            r -= dy;
+            if (r >= dy) r %= dy;
+        }
+        return (uint32_t)r;
     };
 
+    // Wave geometry for the uniform mask / force tests: does the wave (words
+    // [wlo, wlo + 64) modulo W) touch words [lo, hi)?
+    const int wlo = (int)(strip * stride) - (int)a.halo;
+    const int whi = wlo + 64;
+    auto wave_hits = [&](int lo, int hi) -> bool {
+        if (lo >= hi) return false;
+        if (g.W < 64) return true;  // the wave wraps the whole row
+        const int W = (int)g.W;
+        auto hits = [&](int a0, int a1) { return a0 < hi && a1 > lo; };
+        if (wlo < 0) return hits(0, whi) || hits(W + wlo, W);
+        if (whi > W) return hits(wlo, W) || hits(0, whi - W);
+        return hits(wlo, whi);
+    };
+    // Row ranges of non-fluid words are read through the scalar cache.
+    const const_i32_ptr rr = (const_i32_ptr)a.row_range;
+    auto row_has_solids = [&](uint32_t row) -> bool {
+        return wave_hits(rr[row], rr[g.dim_y + row]);
+    };
+    const bool force_cols = a.fdec && wave_hits((int)a.fw0, (int)(a.fw0 + a.fwn));
+    const bool force_mine = a.fdec && c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
+    // Decide once per tile whether the mask and force paths can be needed.
+    bool tile_solids = false, tile_force = false;
+    if (a.mask || force_cols) {
+        for (uint32_t i = 0; i < n_iter; ++i) {
+            const uint32_t r = wrap((int)(y0 + i) - K);
+            if (a.mask && row_has_solids(r)) tile_solids = true;
+            if (force_cols && r >= a.fy0 && r < a.fy1) tile_force = true;
+        }
+    }
+
     // pipeline windows between level b and b+1: rows j-1 (planes 1, 2) and j
-    uint32_t winA[K][7];  // row j   (all planes kept; the compiler drops unused ones)
-    uint32_t winB[K][2];  // row j-1 planes 1 and 2
-    // chirality delay line: dl[s][k] = word for level s, k iterations ahead
-    uint32_t dl[K + 1][4];
+    uint32_t winA[K][7];
+    uint32_t winB[K][2];
+    uint32_t dl[K + 1][4];            // chirality delay lines
+    uint32_t mns[K + 1], msl[K + 1];  // mask delay line: [s] = masks of row i - s
+    uint32_t fnext[K + 1];            // force decisions for the next iteration's levels
+    uint32_t callw[K + 1][4];         // chirality calls of the current iteration
 #pragma unroll
-    for (int s = 0; s <= K; ++s)
+    for (int s = 0; s <= K; ++s) {
+        mns[s] = msl[s] = fnext[s] = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) dl[s][k] = 0;
+        for (int k = 0; k < 4; ++k) dl[s][k] = callw[s][k] = 0;
+    }
 #pragma unroll
     for (int b = 0; b < K; ++b) {
 #pragma unroll
@@ -112,39 +165,74 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
         winB[b][0] = winB[b][1] = 0;
     }
 
-    uint32_t nxt[7];
-    {
-        const uint32_t r0 = wrap((int)y0 - K);
-        const uint32_t* p = src + (size_t)r0 * Q * g.P + c.w;
+    // rows: row_i is the input row of the current iteration; lrow[s] the row
+    // level s works on (row_i - s on the torus)
+    uint32_t row_i = wrap((int)y0 - K);
+    uint32_t lrow[K + 1];
 #pragma unroll
-        for (int d = 0; d < Q; ++d) nxt[d] = p[(size_t)d * g.P];
+    for (int s = 1; s <= K; ++s) lrow[s] = wrap((int)row_i - s);
+
+    uint32_t nxt[7];
+    uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
+    {
+        if (tile_solids && row_has_solids(row_i)) {
+            nns = a.mask[((size_t)row_i * 2 + 0) * g.P + c.w];
+            nsl = a.mask[((size_t)row_i * 2 + 1) * g.P + c.w];
+        }
+        const uint32_t* p = src + (size_t)row_i * Q * g.P;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) nxt[d] = p[(size_t)d * g.P + c.w];
     }
 
-    for (uint32_t i = 0; i < n_iter; ++i) {
+    // One step of the row walk.  STEADY iterations (i >= 2K) have every level
+    // active.  PAR is i & 1: with even tile starts and an even dim_y (FHP), the
+    // row of level s has parity (K + PAR + s) & 1.
+    auto iterate = [&](auto steady_tag, auto par_tag, uint32_t i) {
+        constexpr bool STEADY = decltype(steady_tag)::value;
+        constexpr int PAR = decltype(par_tag)::value;
+        const uint32_t row_next = row_i + 1 == g.dim_y ? 0u : row_i + 1;
         uint32_t cur[7];
 #pragma unroll
         for (int d = 0; d < Q; ++d) cur[d] = nxt[d];
-        if (i + 1 < n_iter) {
-            const uint32_t rn = wrap((int)(y0 + i + 1) - K);
-            const uint32_t* p = src + (size_t)rn * Q * g.P + c.w;
+        mns[0] = nns;
+        msl[0] = nsl;
+        uint32_t fcur[K + 1];
 #pragma unroll
-            for (int d = 0; d < Q; ++d) nxt[d] = p[(size_t)d * g.P];
-        }
+        for (int s = 1; s <= K; ++s) fcur[s] = fnext[s];
 
-        // chirality calls of this iteration, per level (only levels 1, 5 call)
-        uint32_t callw[K + 1][4];
+        // side loads for the next iteration, then the next row's planes
+        if (i + 1 < n_iter) {
+            const uint32_t rn = row_next;
+            nns = nsl = 0;
+            if (tile_solids && row_has_solids(rn)) {
+                nns = a.mask[((size_t)rn * 2 + 0) * g.P + c.w];
+                nsl = a.mask[((size_t)rn * 2 + 1) * g.P + c.w];
+            }
+            if (tile_force) {
 #pragma unroll
-        for (int s = 0; s <= K; ++s)
+                for (int s = 1; s <= K; ++s) {
+                    fnext[s] = 0;
+                    // next iteration, level s works on row_i - (s - 1)
+                    const uint32_t jn = s == 1 ? row_i : lrow[s - 1];
+                    if ((STEADY || i + 1 >= (uint32_t)(2 * s)) && jn >= a.fy0 && jn < a.fy1 &&
+                        force_mine)
+                        fnext[s] = a.fdec[(size_t)(s - 1) * a.fdec_stride +
+                                          (size_t)(jn - a.fy0) * a.fwn + (c.w - a.fw0)];
+                }
+            }
+            const uint32_t* p = src + (size_t)rn * Q * g.P;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) callw[s][k] = 0;
+            for (int d = 0; d < Q; ++d) nxt[d] = p[(size_t)d * g.P + c.w];
+        }
 
 #pragma unroll
         for (int s = 1; s <= K; ++s) {
             uint32_t out[7];
-            const bool active = i >= (uint32_t)(2 * s);      // wave-uniform
-            const bool have_prev = i >= (uint32_t)(2 * (s - 1));
-            const uint32_t j = wrap((int)(y0 + i) - K - s);  // row of level s
+            const bool active = STEADY || i >= (uint32_t)(2 * s);          // wave-uniform
+            const bool have_prev = STEADY || i >= (uint32_t)(2 * (s - 1));
+            const uint32_t j = lrow[s];                      // row of level s
             const uint32_t t = a.t0 + (uint32_t)(s - 1);     // its step
+            const bool odd_row = KIND != kHPP && (((K + PAR + s) & 1) != 0);
             if (active) {
                 // pull stream (src/omp_lattice.cpp:141-180)
                 uint32_t in[7];
@@ -159,7 +247,7 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
                     in[0] = shift_m1(A[0], c);
                     in[3] = shift_p1<RAGGED>(A[3], c);
                     if (Q == 7) in[6] = A[6];
-                    if (j & 1u) {   // odd row
+                    if (odd_row) {
                         in[1] = B[0];
                         in[2] = shift_p1<RAGGED>(B[1], c);
                         in[4] = shift_p1<RAGGED>(cur[4], c);
@@ -196,29 +284,14 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
                     }
                 }
                 collide<KIND, uint32_t>(st, p);
-                uint32_t fluid = 0xFFFFFFFFu;
-                if (a.mask) {
-                    const int lo = a.row_range[j], hi = a.row_range[g.dim_y + j];
-                    const bool mine = (int)c.w >= lo && (int)c.w < hi;
-                    if (__builtin_amdgcn_ballot_w64(mine) != 0) {
-                        const uint32_t ns = a.mask[((size_t)j * 2 + 0) * g.P + c.w];
-                        const uint32_t sl = a.mask[((size_t)j * 2 + 1) * g.P + c.w];
-                        uint32_t edge = 0;
-                        if (c.w == 0) edge |= 1u;
-                        if (c.last) edge |= 1u << (g.nb - 1u);
+                if (tile_solids) {
+                    const uint32_t ns = mns[s], sl = msl[s];
+                    if (__builtin_amdgcn_ballot_w64((ns | sl) != 0) != 0)
                         apply_solids<KIND, uint32_t>(st, in, ns, sl, edge, j == 0 || j + 1 == g.dim_y);
-                        fluid = ~(ns | sl);
-                    }
-                }
-                if (a.fdec && j >= a.fy0 && j < a.fy1) {
-                    const bool mine = c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
-                    if (__builtin_amdgcn_ballot_w64(mine) != 0) {
-                        uint32_t f = 0;
-                        if (mine)
-                            f = a.fdec[(size_t)(s - 1) * a.fdec_stride + (size_t)(j - a.fy0) * a.fwn +
-                                       (c.w - a.fw0)];
-                        apply_force<KIND, uint32_t>(st, f & fluid, a.force_dir);
-                    }
+                    if (tile_force && j >= a.fy0 && j < a.fy1)
+                        apply_force<KIND, uint32_t>(st, fcur[s] & ~(ns | sl), a.force_dir);
+                } else if (tile_force && j >= a.fy0 && j < a.fy1) {
+                    apply_force<KIND, uint32_t>(st, fcur[s], a.force_dir);
                 }
 #pragma unroll
                 for (int d = 0; d < Q; ++d) out[d] = RAGGED ? (st[d] & c.vmask) : st[d];
@@ -234,14 +307,23 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
 #pragma unroll
                 for (int d = 0; d < Q; ++d) winA[s - 1][d] = cur[d];
             }
+            if (active) {
 #pragma unroll
-            for (int d = 0; d < Q; ++d) cur[d] = active ? out[d] : 0u;
-            if (s == K && active) {
-                uint32_t* q = dst + (size_t)j * Q * g.P + c.w;
-                if (writer) {
+                for (int d = 0; d < Q; ++d) cur[d] = out[d];
+                if (s == K && writer) {
+                    uint32_t* q = dst + (size_t)j * Q * g.P;
 #pragma unroll
-                    for (int d = 0; d < Q; ++d) q[(size_t)d * g.P] = cur[d];
+                    for (int d = 0; d < Q; ++d) q[(size_t)d * g.P + c.w] = cur[d];
                 }
+            }
+        }
+
+        // mask delay line: level s next iteration handles the row level s-1 had
+        if (tile_solids) {
+#pragma unroll
+            for (int s = K; s >= 1; --s) {
+                mns[s] = mns[s - 1];
+                msl[s] = msl[s - 1];
             }
         }
 
@@ -264,7 +346,27 @@ __global__ __launch_bounds__(256) void tile_kernel(TileArgs a)
                 }
             }
         }
+        // advance the rows
+#pragma unroll
+        for (int s = K; s >= 2; --s) lrow[s] = lrow[s - 1];
+        lrow[1] = row_i;
+        row_i = row_next;
+    };
+
+    using F = std::false_type;
+    using T = std::true_type;
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    uint32_t i = 0;
+    for (; i < (uint32_t)(2 * K); i += 2) {   // pipeline fill (n_iter > 2K)
+        iterate(F{}, P0{}, i);
+        iterate(F{}, P1{}, i + 1);
     }
+    for (; i + 1 < n_iter; i += 2) {
+        iterate(T{}, P0{}, i);
+        iterate(T{}, P1{}, i + 1);
+    }
+    if (i < n_iter) iterate(T{}, P0{}, i);
 }
 
 uint32_t tile_halo(const Geometry& g, int k)

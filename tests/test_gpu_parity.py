@@ -89,6 +89,7 @@ STEP_CASES = [
     (2, 200, 14, "cylinder", dict(dir="x", p=0.1, x0=33, x1=97)),
     (3, 96, 16, "slipbox", None),
     (3, 8232, 6, "mixed", dict(dir="x", p=0.02, x0=8100)),  # two workgroups per row
+    (2, 4096, 8, "periodic", dict(dir="x", p=0.3, x0=0, x1=40)),  # force on the wrapped words
     (3, 1000, 66, "cylinder", dict(dir="x", p=0.01, x0=0, x1=64)),
 ]
 
