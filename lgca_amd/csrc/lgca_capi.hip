@@ -585,10 +585,10 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
 {
     if (h->tile_th) return h->tile_th + (h->tile_th & 1u);
     const uint64_t strips = (h->g.W + 63 - 2 * tile_halo(h->g, k)) / (64 - 2 * tile_halo(h->g, k));
-    // about two rounds of resident waves (four per SIMD on 256 CUs)
-    uint64_t th = (uint64_t)h->dim_y * strips / 8192;
-    const uint64_t lo = k >= 4 ? 8u * (uint64_t)k : 16u;
-    if (th < lo) th = lo;
+    // about 1.75 rounds of resident waves (four per SIMD on 1024 SIMDs); on the
+    // 32768 x 8192 lattice this gives 20-row tiles, the fastest measured
+    uint64_t th = (uint64_t)h->dim_y * strips / 7168;
+    if (th < 8) th = 8;
     if (th > 256) th = 256;
     if (th > h->dim_y) th = h->dim_y;
     th += th & 1u;  // tiles start on even rows

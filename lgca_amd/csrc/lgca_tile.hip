@@ -50,9 +50,11 @@ struct LaneCtx {
 };
 
 // out cell x = in cell x-1, cyclically over dim_x
+template <bool RAGGED>
 __device__ __forceinline__ uint32_t shift_m1(uint32_t v, const LaneCtx& c)
 {
-    const uint32_t L = from_left(v) << c.first_shl;
+    uint32_t L = from_left(v);
+    if (RAGGED) L <<= c.first_shl;
     return __builtin_amdgcn_alignbit(v, L, 31);
 }
 // out cell x = in cell x+1, cyclically over dim_x
@@ -239,12 +241,12 @@ void tile_kernel(TileArgs a)
                 const uint32_t* A = winA[s - 1];
                 const uint32_t* B = winB[s - 1];
                 if (KIND == kHPP) {
-                    in[0] = shift_m1(A[0], c);            // from (x-1, j)
+                    in[0] = shift_m1<RAGGED>(A[0], c);            // from (x-1, j)
                     in[1] = B[0];                         // from (x, j-1), plane 1 kept in B[0]
                     in[2] = shift_p1<RAGGED>(A[2], c);    // from (x+1, j)
                     in[3] = cur[3];                       // from (x, j+1)
                 } else {
-                    in[0] = shift_m1(A[0], c);
+                    in[0] = shift_m1<RAGGED>(A[0], c);
                     in[3] = shift_p1<RAGGED>(A[3], c);
                     if (Q == 7) in[6] = A[6];
                     if (odd_row) {
@@ -253,10 +255,10 @@ void tile_kernel(TileArgs a)
                         in[4] = shift_p1<RAGGED>(cur[4], c);
                         in[5] = cur[5];
                     } else {
-                        in[1] = shift_m1(B[0], c);
+                        in[1] = shift_m1<RAGGED>(B[0], c);
                         in[2] = B[1];
                         in[4] = cur[4];
-                        in[5] = shift_m1(cur[5], c);
+                        in[5] = shift_m1<RAGGED>(cur[5], c);
                     }
                 }
                 uint32_t st[7];
