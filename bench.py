@@ -130,9 +130,11 @@ def main() -> int:
     sups = n * args.steps / (ms / 1e3)
     bytes_per_step = n * 2 * Q[w["model"]] / 8
     achieved = bytes_per_step * args.steps / (ms / 1e3) / 1e9
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE for the default launch shape (see the file)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"step_kernel_{args.workload}_traffic.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and args.steps_per_pass is None and args.tile_rows == 0:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     rec = {
