@@ -74,3 +74,19 @@ def test_disc_matches_float_rounded_distance():
     assert (d < 5.5).all()
     assert m.sum() == sum(1 for y in range(32) for x in range(64)
                           if np.float32(np.sqrt((x - 20) ** 2 + (y - 16) ** 2)) < 5.5)
+
+
+def test_cpp_dropin_compiles_against_the_c_abi():
+    """The C++ drop-in header and the viewer-loop program compile with a plain
+    host compiler against include/lgca.h alone (no HIP headers)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = os.path.join(ROOT, "tests", "native", "viewer_loop.cpp")
+    r = subprocess.run([gxx, "-std=c++11", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROOT, "lgca_amd", "host"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
