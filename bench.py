@@ -1,12 +1,27 @@
 """Benchmark of the lattice-gas step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload karman|pipe|hpp]
+                    [--ensemble-jobs 512] [--serve-rates 50,200,1000,2000]
 
 Default workload: FHP-III Karman street, 32768 x 8192 cells (268,435,456),
 NO_SLIP cylinder of diameter 512 at (4096, 4099), walls on rows 0 and 8191,
 fill 0.25 per direction, per-cell x force p = 2e-3 on x < 64, hash seed 11.
 A step is one collide-and-propagate time step of the whole lattice; K steps
 are timed after W warm-up steps, with the state resident in HBM throughout.
+
+Two more measurements ride on the same JSON line:
+  "ensemble": 512 FHP-II jobs (long side 128..2048 step 64, aspect 1..4,
+      periodic / pipe / cylinder of radius 8..L/8, fill 0.1-0.5, own seed,
+      1000-10000 steps, 32x32 coarse fields) submitted at once to one batch;
+      jobs/s against the single-lattice path run one job at a time (timed on
+      an evenly spaced sample of the same jobs), with bit-equality of sampled
+      jobs between the two.
+  "serving": Poisson arrivals at each rate of jobs uniform over HPP, FHP-I,
+      FHP-II, FHP-III, 256x256..4096x1024, periodic / pipe (x force p
+      1e-4..5e-3) / cylinder (random radius and position), 500-20000 steps,
+      own seed, admitted against a 200 GB budget; p50/p99 latency and jobs/s,
+      next to the one-at-a-time path (each job's measured single-lattice
+      service time, queued first-come first-served on the same arrivals).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -86,6 +101,165 @@ def cpu_baseline(w, lat, step0: int, budget_s: float):
                       f"step {step0}, {t1:.1f} s, oracle/cpu_ref.c with {threads} OpenMP threads"}
 
 
+# ------------------------------------------------------------ ensembles
+
+def c4_jobs(n: int, seed: int = 2024):
+    """The batched-ensemble workload: FHP-II lattices with long side L in
+    {128..2048} step 64, aspect 1..4 (short side rounded to 64, at least 64),
+    either orientation, periodic / pipe / cylinder (radius 8..L/8 at
+    (dim_x/4, dim_y/2)), fill U(0.1, 0.5), own seed, 1000-10000 steps, r = 16."""
+    import numpy as np
+    from lgca_amd import Job
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        L = int(rng.integers(2, 33)) * 64
+        S = max(64, int(round(L / float(rng.uniform(1.0, 4.0)) / 64.0)) * 64)
+        dx, dy = (L, S) if rng.random() < 0.5 else (S, L)
+        geom = ("periodic", "pipe", "cylinder")[int(rng.integers(0, 3))]
+        rad = int(rng.integers(8, L // 8 + 1))
+        jobs.append(Job("FHP_II", dx, dy, int(rng.integers(1000, 10001)), seed=100000 + i,
+                        coarse_radius=16, fill=float(rng.uniform(0.1, 0.5)), geometry=geom,
+                        cyl_x=dx // 4, cyl_y=dy // 2, cyl_diameter=2.0 * rad))
+    return jobs
+
+
+def c5_jobs(n: int, seed: int = 2025):
+    """The serving workload: model uniform over the four, dim_x in 256..4096
+    and dim_y in 256..1024 (multiples of 64), periodic / pipe with x force
+    p U(1e-4, 5e-3) / cylinder (walls plus a disc of random radius 8..dim_y/8
+    at a random position), fill 0.25, 500-20000 steps, own seed, r = 16."""
+    import numpy as np
+    from lgca_amd import Job
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        model = int(rng.integers(0, 4))
+        dx, dy = int(rng.integers(4, 65)) * 64, int(rng.integers(4, 17)) * 64
+        geom = ("periodic", "pipe", "cylinder")[int(rng.integers(0, 3))]
+        j = Job(model, dx, dy, int(rng.integers(500, 20001)), seed=200000 + i, coarse_radius=16,
+                fill=0.25, geometry=geom)
+        if geom == "pipe":
+            j.force_dir, j.force_p = "x", float(rng.uniform(1e-4, 5e-3))
+        elif geom == "cylinder":
+            rad = int(rng.integers(8, dy // 8 + 1))
+            j.cyl_diameter = 2.0 * rad
+            j.cyl_x = int(rng.integers(rad, dx - rad))
+            j.cyl_y = int(rng.integers(rad, dy - rad))
+        jobs.append(j)
+    return jobs
+
+
+def single_lattice_run(job, want_state: bool = False):
+    """One job through the plain single-lattice path, start to finish: create,
+    set-up, fill, steps, post-process, results, destroy.  Returns (seconds,
+    state bytes or None, coarse density, coarse momentum)."""
+    from lgca_amd import Lattice
+    t = time.perf_counter()
+    lat = Lattice(job.model, dim_x=job.dim_x, dim_y=job.dim_y,
+                  coarse_graining_radius=job.coarse_radius, seed=job.seed)
+    if job.geometry == "pipe":
+        lat.apply_bc_pipe()
+    elif job.geometry == "cylinder":
+        lat.apply_bc_cylinder(job.cyl_x, job.cyl_y, job.cyl_diameter)
+    lat.init_random(job.fill)
+    if job.force_dir is not None:
+        lat.set_forcing(job.force_dir, job.force_p)
+    lat.step(job.steps)
+    lat.copy_data_to_output_buffer()
+    lat.post_process(cells=False)
+    md, mm = lat.mean_density().copy(), lat.mean_momentum().copy()
+    st = lat.get_state_u8() if want_state else None
+    lat.synchronize()
+    lat.close()
+    return time.perf_counter() - t, st, md, mm
+
+
+def ensemble_bench(n_jobs: int, sample: int):
+    import numpy as np
+    from lgca_amd import Batch
+    jobs = c4_jobs(n_jobs)
+    su = sum(j.dim_x * j.dim_y * j.steps for j in jobs)
+    with Batch(hbm_budget=200 * 10 ** 9) as b:
+        t0 = time.perf_counter()
+        ids = [b.submit(j) for j in jobs]
+        b.wait(timeout=600)
+        t_batch = time.perf_counter() - t0
+        stats = b.stats()
+        picks = list(range(0, n_jobs, max(1, n_jobs // max(1, sample))))[:sample]
+        got = {i: b.result(ids[i]) for i in picks[:4]}
+    t_one, identical = 0.0, True
+    for k, i in enumerate(picks):
+        dt, st, md, mm = single_lattice_run(jobs[i], want_state=i in got)
+        t_one += dt
+        if i in got:
+            gst, gmd, gmm = got[i]
+            identical &= bool(np.array_equal(gst, st) and np.array_equal(gmd.view(np.uint32),
+                                                                         md.view(np.uint32))
+                              and np.array_equal(gmm.view(np.uint32), mm.view(np.uint32)))
+    batch_rate = n_jobs / t_batch
+    one_rate = len(picks) / t_one
+    return {
+        "jobs": n_jobs, "site_updates": su, "seconds": round(t_batch, 3),
+        "jobs_per_s": round(batch_rate, 2), "gsup_per_s": round(su / t_batch / 1e9, 1),
+        "one_at_a_time_jobs_per_s": round(one_rate, 2),
+        "one_at_a_time_sample": len(picks), "speedup": round(batch_rate / one_rate, 2),
+        "bits_identical_sampled": identical, "rounds": stats["rounds"],
+        "launches": stats["launches"], "tuned_steps_per_pass": stats["steps_per_pass"],
+        "tuned_wave_target": stats["wave_target"],
+        "workload": "512 FHP-II lattices, L 128..2048, aspect 1..4, periodic/pipe/cylinder, "
+                    "fill 0.1-0.5, 1000-10000 steps" if n_jobs == 512 else f"{n_jobs} C4-style jobs",
+    }
+
+
+def serving_bench(rates, n_jobs: int, max_active_cells: int):
+    import numpy as np
+    from lgca_amd import Batch
+    jobs = c5_jobs(n_jobs)
+    rng = np.random.default_rng(7)
+    unit_arrivals = np.cumsum(rng.exponential(1.0, n_jobs))
+    unit_arrivals -= unit_arrivals[0]
+    # one-at-a-time service times, measured once per job
+    service = np.array([single_lattice_run(j)[0] for j in jobs])
+    out = []
+    for lam in rates:
+        arr = unit_arrivals / lam
+        with Batch(hbm_budget=200 * 10 ** 9) as b:
+            b.set_tuning(0, 0, max_active_cells)
+            ids = []
+            t0 = time.monotonic()
+            for i, j in enumerate(jobs):
+                delay = t0 + arr[i] - time.monotonic()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(b.submit(j))
+            b.wait(timeout=900)
+            infos = [b.info(i) for i in ids]
+        lat = np.array([x["done_s"] - x["submit_s"] for x in infos])
+        span = max(x["done_s"] for x in infos) - min(x["submit_s"] for x in infos)
+        # one at a time: the same arrivals served first-come first-served
+        fin = np.empty(n_jobs)
+        prev = -np.inf
+        for i in range(n_jobs):
+            prev = max(arr[i], prev) + service[i]
+            fin[i] = prev
+        lat1 = fin - arr
+        out.append({
+            "rate": lam, "jobs": n_jobs,
+            "jobs_per_s": round(n_jobs / span, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "one_at_a_time_jobs_per_s": round(n_jobs / (fin[-1] - arr[0]), 2),
+            "one_at_a_time_p50_ms": round(float(np.percentile(lat1, 50)) * 1e3, 2),
+            "one_at_a_time_p99_ms": round(float(np.percentile(lat1, 99)) * 1e3, 2),
+        })
+        print(f"[serving] rate {lam}: {out[-1]}", file=sys.stderr, flush=True)
+    return {"levels": out, "max_active_cells": max_active_cells,
+            "mean_service_ms_one_at_a_time": round(float(service.mean()) * 1e3, 2),
+            "workload": "Poisson arrivals; HPP/FHP-I/FHP-II/FHP-III, 256x256..4096x1024, "
+                        "periodic/pipe(force)/cylinder, 500-20000 steps, 200 GB budget"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +271,11 @@ def main() -> int:
     ap.add_argument("--steps-per-pass", type=int, default=None,
                     help="steps fused per HBM pass (1, 2, 4, 8; 0 = one-step quad kernel)")
     ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--ensemble-jobs", type=int, default=512, help="0 skips the ensemble leg")
+    ap.add_argument("--ensemble-sample", type=int, default=32)
+    ap.add_argument("--serve-rates", default="50,200,1000,2000", help="empty skips serving")
+    ap.add_argument("--serve-jobs", type=int, default=120)
+    ap.add_argument("--serve-max-active-cells", type=int, default=1 << 28)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -158,12 +337,22 @@ def main() -> int:
         "particles_conserved": n0 == n1,
         "wall_s": round(wall, 3),
     }
-    if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(w, lat, args.warmup + args.steps, args.cpu_seconds)
     lat.close()
     start.close()
     end.close()
     stream.close()
+    if args.ensemble_jobs > 0:
+        print("[bench] ensemble", file=sys.stderr, flush=True)
+        rec["ensemble"] = ensemble_bench(args.ensemble_jobs, args.ensemble_sample)
+    rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
+    if rates:
+        print("[bench] serving", file=sys.stderr, flush=True)
+        rec["serving"] = serving_bench(rates, args.serve_jobs, args.serve_max_active_cells)
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        lat = build_lattice(w)
+        rec["cpu_baseline"] = cpu_baseline(w, lat, 0, args.cpu_seconds)
+        lat.close()
     print(json.dumps(rec), flush=True)
     return 0 if n0 == n1 else 1
 

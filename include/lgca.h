@@ -164,6 +164,100 @@ LGCA_API lgca_status lgca_count_particles(lgca_lattice* h, uint64_t* count);
  * get_mean_velocity (src/omp_lattice.cpp:508-557).  Summed exactly. */
 LGCA_API lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2]);
 
+/* ------------------------------------------------------------------------
+ * Batched ensembles and serving.  No counterpart in the reference, which runs
+ * one lattice per process: a job here is one run of that lattice, set up as
+ * apply_bc_periodic / apply_bc_pipe / apply_bc_karman_vortex_street
+ * (src/lattice.cpp:221-280) or from given cell types, filled as init_random
+ * (src/lattice.cpp:198-217), advanced `steps` collide-and-propagate steps
+ * (src/omp_lattice.cpp:100-249) with the optional per-cell force, and
+ * post-processed (src/omp_lattice.cpp:349-454).  Every job's bits are the ones
+ * the single-lattice entry points above give for the same parameters.
+ *
+ * A batch owns one GPU stream and a worker thread.  Jobs are submitted at any
+ * time; the worker admits them in order while their device memory fits the
+ * batch's HBM budget (and their cells the active-cell cap), and advances
+ * all admitted jobs together, a few steps per round, in one kernel launch per
+ * (model, steps per pass, ragged width) class.  A finished job's outputs stay
+ * on the GPU until released.  All batch entry points are thread-safe.
+ * ---------------------------------------------------------------------- */
+typedef struct lgca_batch lgca_batch;
+
+enum { LGCA_GEOM_PERIODIC = 0, LGCA_GEOM_PIPE = 1, LGCA_GEOM_CYLINDER = 2, LGCA_GEOM_TYPES = 3 };
+enum { LGCA_JOB_QUEUED = 0, LGCA_JOB_RUNNING = 1, LGCA_JOB_DONE = 2, LGCA_JOB_RELEASED = 3,
+       LGCA_JOB_FAILED = 4 };
+#define LGCA_ALL_JOBS UINT64_MAX
+
+typedef struct lgca_job_desc {
+    int32_t model;             /* LGCA_HPP .. LGCA_FHP_III                             */
+    uint32_t dim_x, dim_y;     /* as lgca_create                                       */
+    uint32_t coarse_radius;    /* r of the coarse fields (window 2r)                   */
+    uint64_t seed;             /* hash key of this job                                 */
+    uint64_t steps;            /* steps to run                                         */
+    double fill;               /* per-bit fill of FLUID cells (ignored if `state`)     */
+    int32_t geometry;          /* LGCA_GEOM_*: all FLUID; NO_SLIP rows 0 and dim_y-1;
+                                  those plus a NO_SLIP disc; or `cell_types`          */
+    int32_t cyl_x, cyl_y;      /* disc centre                                           */
+    double cyl_diameter;       /* disc: (float)sqrt(dx^2 + dy^2) < diameter / 2,
+                                  as src/lattice.cpp:264-279                          */
+    int32_t force_dir;         /* per-cell force as lgca_set_forcing                   */
+    double force_p;
+    uint32_t fx0, fx1, fy0, fy1;
+    const uint8_t* cell_types; /* LGCA_GEOM_TYPES: num_cells bytes, copied at submit   */
+    const uint8_t* state;      /* initial state bytes (copied at submit) or NULL       */
+} lgca_job_desc;
+
+typedef struct lgca_job_info {
+    int32_t status;            /* LGCA_JOB_*                                           */
+    uint32_t reserved;
+    uint64_t step;             /* steps done (exact once DONE)                         */
+    uint64_t device_bytes;     /* what the job holds against the budget               */
+    double submit_s, admit_s, done_s;  /* host clock (seconds, monotonic); 0 = not yet */
+} lgca_job_info;
+
+typedef struct lgca_batch_stats {
+    uint64_t jobs_submitted, jobs_done, jobs_active, jobs_queued;
+    uint64_t rounds;           /* rounds queued                                        */
+    uint64_t launches;         /* step-kernel launches queued                          */
+    uint64_t site_updates;     /* cells x steps queued                                 */
+    uint64_t bytes_in_use;     /* device bytes of admitted, unreleased jobs            */
+    uint64_t arena_bytes;      /* device memory reserved by the batch                  */
+    int32_t steps_per_pass;    /* current tuning                                       */
+    uint32_t wave_target;      /* waves per round the tile height aims at              */
+    uint64_t tunings;          /* autotuner decisions taken                            */
+} lgca_batch_stats;
+
+/* Creates a batch on GPU `device` whose jobs may hold at most
+ * hbm_budget_bytes of device memory together. */
+LGCA_API lgca_status lgca_batch_create(lgca_batch** out, int device, uint64_t hbm_budget_bytes);
+/* Stops the worker (unfinished jobs are abandoned) and frees everything. */
+LGCA_API lgca_status lgca_batch_destroy(lgca_batch* b);
+LGCA_API const char* lgca_batch_last_error(const lgca_batch* b);
+/* Device bytes the job would hold (what admission checks against the budget). */
+LGCA_API lgca_status lgca_batch_job_bytes(const lgca_job_desc* desc, uint64_t* bytes);
+/* Validates and queues a job; *job_id identifies it from then on.  Fails with
+ * LGCA_ERR_BUDGET if the job alone exceeds the budget. */
+LGCA_API lgca_status lgca_batch_submit(lgca_batch* b, const lgca_job_desc* desc, uint64_t* job_id);
+/* Waits until the job (or every submitted job, LGCA_ALL_JOBS) is DONE, at
+ * most timeout_s seconds (< 0: no limit).  LGCA_ERR_INVALID_STATE on time-out. */
+LGCA_API lgca_status lgca_batch_wait(lgca_batch* b, uint64_t job_id, double timeout_s);
+/* Ids of jobs that became DONE since the last poll, in completion order. */
+LGCA_API lgca_status lgca_batch_poll(lgca_batch* b, uint64_t* ids, uint32_t max_ids, uint32_t* n);
+LGCA_API lgca_status lgca_batch_job_info(lgca_batch* b, uint64_t job_id, lgca_job_info* info);
+/* Outputs of a DONE job into host memory (any may be NULL): the state bytes
+ * (num_cells), coarse density (Nc) and coarse momentum (2Nc, x/y
+ * interleaved) of its final state. */
+LGCA_API lgca_status lgca_batch_result(lgca_batch* b, uint64_t job_id, uint8_t* state,
+                                       float* mean_density, float* mean_momentum);
+/* Returns a DONE job's device memory to the budget. */
+LGCA_API lgca_status lgca_batch_release(lgca_batch* b, uint64_t job_id);
+/* Launch shape: steps per pass (2, 4 or 8; 0 = autotune), waves per round the
+ * tile height aims at (0 = autotune), and the cap on cells of running jobs
+ * (0 = none).  Results do not depend on it. */
+LGCA_API lgca_status lgca_batch_set_tuning(lgca_batch* b, int steps_per_pass, uint32_t wave_target,
+                                           uint64_t max_active_cells);
+LGCA_API lgca_status lgca_batch_get_stats(lgca_batch* b, lgca_batch_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,7 @@ load the library; the first call that needs it does, and fails loudly when the
 library has not been built.
 """
 from ._capi import LgcaError, lib
+from .batch import Batch, Job, job_bytes
 from .lattice import Lattice, derive_parameters, disc
 
-__all__ = ["Lattice", "LgcaError", "derive_parameters", "disc", "lib"]
+__all__ = ["Batch", "Job", "Lattice", "LgcaError", "derive_parameters", "disc", "job_bytes", "lib"]

@@ -33,6 +33,40 @@ class Info(C.Structure):
     ]
 
 
+GEOM_PERIODIC, GEOM_PIPE, GEOM_CYLINDER, GEOM_TYPES = range(4)
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED, JOB_FAILED = range(5)
+ALL_JOBS = 2 ** 64 - 1
+
+
+class JobDesc(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("dim_x", C.c_uint32), ("dim_y", C.c_uint32),
+        ("coarse_radius", C.c_uint32), ("seed", C.c_uint64), ("steps", C.c_uint64),
+        ("fill", C.c_double), ("geometry", C.c_int32), ("cyl_x", C.c_int32),
+        ("cyl_y", C.c_int32), ("cyl_diameter", C.c_double), ("force_dir", C.c_int32),
+        ("force_p", C.c_double), ("fx0", C.c_uint32), ("fx1", C.c_uint32),
+        ("fy0", C.c_uint32), ("fy1", C.c_uint32), ("cell_types", C.c_void_p),
+        ("state", C.c_void_p),
+    ]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("reserved", C.c_uint32), ("step", C.c_uint64),
+        ("device_bytes", C.c_uint64), ("submit_s", C.c_double), ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("jobs_submitted", C.c_uint64), ("jobs_done", C.c_uint64), ("jobs_active", C.c_uint64),
+        ("jobs_queued", C.c_uint64), ("rounds", C.c_uint64), ("launches", C.c_uint64),
+        ("site_updates", C.c_uint64), ("bytes_in_use", C.c_uint64), ("arena_bytes", C.c_uint64),
+        ("steps_per_pass", C.c_int32), ("wave_target", C.c_uint32), ("tunings", C.c_uint64),
+    ]
+
+
 SIGNATURES = {
     "lgca_create": [C.POINTER(_h), C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int],
     "lgca_destroy": [_h],
@@ -59,7 +93,20 @@ SIGNATURES = {
                                  C.POINTER(C.c_void_p)],
     "lgca_count_particles": [_h, C.POINTER(C.c_uint64)],
     "lgca_mean_velocity": [_h, _f32p],
+    "lgca_batch_create": [C.POINTER(_h), C.c_int, C.c_uint64],
+    "lgca_batch_destroy": [_h],
+    "lgca_batch_job_bytes": [C.POINTER(JobDesc), C.POINTER(C.c_uint64)],
+    "lgca_batch_submit": [_h, C.POINTER(JobDesc), C.POINTER(C.c_uint64)],
+    "lgca_batch_wait": [_h, C.c_uint64, C.c_double],
+    "lgca_batch_poll": [_h, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)],
+    "lgca_batch_job_info": [_h, C.c_uint64, C.POINTER(JobInfo)],
+    "lgca_batch_result": [_h, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p],
+    "lgca_batch_release": [_h, C.c_uint64],
+    "lgca_batch_set_tuning": [_h, C.c_int, C.c_uint32, C.c_uint64],
+    "lgca_batch_get_stats": [_h, C.POINTER(BatchStats)],
 }
+# entry points returning text rather than a status
+TEXT_ENTRY_POINTS = ("lgca_last_error", "lgca_batch_last_error")
 
 _lib = None
 
@@ -82,8 +129,10 @@ def lib() -> C.CDLL:
             fn = getattr(L, name)
             fn.argtypes = argtypes
             fn.restype = C.c_int
-        L.lgca_last_error.argtypes = [_h]
-        L.lgca_last_error.restype = C.c_char_p
+        for name in TEXT_ENTRY_POINTS:
+            fn = getattr(L, name)
+            fn.argtypes = [_h]
+            fn.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -91,4 +140,10 @@ def lib() -> C.CDLL:
 def check(code: int, handle=None) -> None:
     if code != OK:
         msg = lib().lgca_last_error(handle)
+        raise LgcaError(code, msg.decode() if msg else "")
+
+
+def check_batch(code: int, batch=None) -> None:
+    if code != OK:
+        msg = lib().lgca_batch_last_error(batch)
         raise LgcaError(code, msg.decode() if msg else "")

@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblgca_hip.so")
 
-SOURCES = ["lgca_kernels.hip", "lgca_tile.hip", "lgca_capi.hip"]
+SOURCES = ["lgca_kernels.hip", "lgca_tile.hip", "lgca_capi.hip", "lgca_batch.hip"]
 ARCH = os.environ.get("LGCA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

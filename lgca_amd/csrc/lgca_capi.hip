@@ -100,13 +100,6 @@ hipError_t dalloc(T** p, size_t bytes)
     return e;
 }
 
-uint64_t threshold(double p)
-{
-    if (!(p > 0.0)) return 0;
-    if (p >= 1.0) return (uint64_t)1 << 32;
-    return (uint64_t)std::floor(p * 4294967296.0);
-}
-
 lgca_status ensure_bytes(lgca_lattice* h)
 {
     if (!h->bytes) HIPCHK(h, dalloc(&h->bytes, (size_t)h->dim_x * h->dim_y));
@@ -495,7 +488,7 @@ extern "C" lgca_status lgca_init_random(lgca_lattice* h, double p_fill)
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
     HIPCHK(h, launch_init_fill(h->state[h->cur], h->mask, nullptr, h->g, (uint32_t)h->seed,
-                               (uint32_t)(h->seed >> 32), threshold(p_fill), h->stream));
+                               (uint32_t)(h->seed >> 32), hash_threshold(p_fill), h->stream));
     return LGCA_OK;
 }
 
@@ -510,7 +503,7 @@ extern "C" lgca_status lgca_init_random_masked(lgca_lattice* h, double p_fill,
     HIPCHK(h, hipMemcpyAsync(h->bytes, host_region, (size_t)h->dim_x * h->dim_y,
                              hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, launch_init_fill(h->state[h->cur], h->mask, h->bytes, h->g, (uint32_t)h->seed,
-                               (uint32_t)(h->seed >> 32), threshold(p_fill), h->stream));
+                               (uint32_t)(h->seed >> 32), hash_threshold(p_fill), h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return LGCA_OK;
 }
@@ -527,7 +520,7 @@ extern "C" lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint
         return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_forcing: p must lie in [0, 1]");
     std::lock_guard<std::mutex> lk(h->mu);
     h->force_dir = dir;
-    h->force_thr = threshold(p);
+    h->force_thr = hash_threshold(p);
     h->fx0 = x0;
     h->fx1 = x1 < h->dim_x ? x1 : h->dim_x;
     h->fy0 = y0;

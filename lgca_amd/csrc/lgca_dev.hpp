@@ -170,4 +170,26 @@ __host__ __device__ __forceinline__ void apply_force(T s[7], T f, int dir)
     }
 }
 
+// Force decisions of one 32-cell word for one step (hash stream 1): bit b
+// fires when cell x = 32w + b lies in [fx0, fx1) and x < dim_x, and its draw
+// {x >> 2, y, step, 1}[x & 3] is below thr.
+__device__ __forceinline__ uint32_t force_draw_word(uint32_t w, uint32_t y, uint32_t step,
+                                                    uint32_t key0, uint32_t key1, uint64_t thr,
+                                                    uint32_t fx0, uint32_t fx1, uint32_t dim_x)
+{
+    uint32_t word = 0;
+    for (uint32_t gi = 0; gi < 8; ++gi) {
+        const uint32_t x = 32 * w + 4 * gi;
+        if (x + 4 <= fx0 || x >= fx1 || x >= dim_x) continue;
+        const U4 r = philox4x32_10(x >> 2, y, step, 1u, key0, key1);
+        const uint32_t v[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t xx = x + j;
+            if (xx >= fx0 && xx < fx1 && xx < dim_x && (uint64_t)v[j] < thr) word |= 1u << (4 * gi + j);
+        }
+    }
+    return word;
+}
+
 }  // namespace lgca

@@ -226,22 +226,8 @@ __global__ __launch_bounds__(256) void force_draw_kernel(ForceDrawArgs a)
     const uint32_t ww = (uint32_t)(idx % a.fwn);
     const uint32_t yy = (uint32_t)((idx / a.fwn) % rows);
     const uint32_t k = (uint32_t)(idx / ((uint64_t)a.fwn * rows));
-    const uint32_t w = a.fw0 + ww;
-    const uint32_t y = a.fy0 + yy;
-    const uint32_t step = a.step0 + k;
-    uint32_t word = 0;
-    for (uint32_t gi = 0; gi < 8; ++gi) {
-        const uint32_t x = 32 * w + 4 * gi;
-        if (x + 4 <= a.fx0 || x >= a.fx1 || x >= a.dim_x) continue;
-        const U4 r = philox4x32_10(x >> 2, y, step, 1u, a.key0, a.key1);
-        const uint32_t v[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-            const uint32_t xx = x + j;
-            if (xx >= a.fx0 && xx < a.fx1 && xx < a.dim_x && (uint64_t)v[j] < a.thr)
-                word |= 1u << (4 * gi + j);
-        }
-    }
+    const uint32_t word = force_draw_word(a.fw0 + ww, a.fy0 + yy, a.step0 + k, a.key0, a.key1, a.thr,
+                                          a.fx0, a.fx1, a.dim_x);
     a.dec[((size_t)k * (a.fy1 - a.fy0) + yy) * a.fwn + ww] = word;
 }
 

@@ -61,6 +61,35 @@ struct TileArgs {
 uint32_t tile_halo(const Geometry& g, int k);  // smallest valid TileArgs::halo
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s);
 
+// Batched ensembles.  BatchJob is a job's static descriptor (its TileArgs
+// without src/dst/t0/th/halo, which come from BatchState and WorkItem);
+// BatchState is the job's dynamic state on the device; a WorkItem is one
+// wave's tile.
+struct BatchJob {
+    TileArgs a;            // geometry, masks, keys, force region and window base
+    uint32_t* buf0;
+    uint32_t* buf1;
+    uint64_t force_thr;    // per-cell force: fire when draw < force_thr
+    uint32_t fx0, fx1;     // forced columns [fx0, fx1)
+};
+struct BatchState {
+    uint32_t step;         // absolute step (mod 2^32)
+    uint32_t cur;          // 0: state in buf0, 1: in buf1
+    uint32_t fdec_step0;   // step of the first decision plane at a.fdec
+    uint32_t pad;
+};
+struct WorkItem {
+    uint32_t job, strip, tile;
+    uint32_t shape;        // rows per tile | halo << 16
+};
+// One class launch of a round: every item (all of one model, K and raggedness)
+// advances its job K steps reading its BatchState.  launch_advance then moves
+// each listed job (entry = slot | k << 24) k steps on and swaps its buffers.
+hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
+                             const BatchState* state, const WorkItem* items, uint32_t n_items,
+                             hipStream_t s);
+hipError_t launch_advance(BatchState* state, const uint32_t* entries, uint32_t n, hipStream_t s);
+
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]
     uint32_t nsteps;
@@ -71,6 +100,14 @@ struct ForceDrawArgs {
     uint32_t dim_x;
 };
 hipError_t launch_force_draw(const ForceDrawArgs& a, hipStream_t s);
+
+// Hash threshold of probability p: draws (uniform 32-bit) below it fire.
+inline uint64_t hash_threshold(double p)
+{
+    if (!(p > 0.0)) return 0;
+    if (p >= 1.0) return (uint64_t)1 << 32;
+    return (uint64_t)(p * 4294967296.0);  // p * 2^32 < 2^32: truncation is floor
+}
 
 hipError_t launch_pack(const uint8_t* state, uint32_t* planes, const Geometry& g,
                        uint32_t* err, hipStream_t s);

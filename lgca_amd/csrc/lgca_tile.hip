@@ -74,20 +74,19 @@ typedef const __attribute__((address_space(4))) int32_t* const_i32_ptr;
 // K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
 template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
 
+// One wave's work: x-strip `strip` of row tile `tile` of the lattice in `a`.
 template <int KIND, int K, bool RAGGED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
-void tile_kernel(TileArgs a)
+__device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile)
 {
     constexpr int Q = num_dir_of_kind(KIND);
     const Geometry& g = a.g;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t strip = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t stride = 64u - 2u * a.halo;
     const uint32_t nstrips = (g.W + stride - 1) / stride;
     if (strip >= nstrips) return;  // whole wave
     // tiles start on even rows (a.th is even), so the parity of the row a
     // level works on follows from the iteration index: see `odd_row` below
-    const uint32_t y0 = blockIdx.y * a.th;
+    const uint32_t y0 = tile * a.th;
     const uint32_t rows = min(a.th, g.dim_y - y0);
 
     const int wabs = (int)(strip * stride) - (int)a.halo + (int)lane;
@@ -369,6 +368,79 @@ void tile_kernel(TileArgs a)
         iterate(T{}, P1{}, i + 1);
     }
     if (i < n_iter) iterate(T{}, P0{}, i);
+}
+
+template <int KIND, int K, bool RAGGED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+void tile_kernel(TileArgs a)
+{
+    tile_body<KIND, K, RAGGED>(a, __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)),
+                               blockIdx.y);
+}
+
+// Batched form: every wave takes one work item (job, strip, tile, shape) and
+// runs that job's lattice.  A job's step, current buffer and force-window
+// origin live in device memory (advance_kernel moves them after each round),
+// so queued rounds need no host-to-device traffic.  One launch covers the
+// items of one (model, K, ragged) class.
+template <int KIND, int K, bool RAGGED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __restrict__ state,
+                       const WorkItem* __restrict__ items, uint32_t n_items)
+{
+    // wave-uniform by construction; readfirstlane lets the compiler keep the
+    // job record in scalar registers
+    const uint32_t wi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (wi >= n_items) return;
+    const WorkItem it = items[wi];
+    const BatchState bs = state[it.job];
+    TileArgs a = jobs[it.job].a;
+    a.t0 = bs.step;
+    a.th = it.shape & 0xFFFFu;
+    a.halo = it.shape >> 16;
+    a.src = bs.cur ? jobs[it.job].buf1 : jobs[it.job].buf0;
+    a.dst = bs.cur ? jobs[it.job].buf0 : jobs[it.job].buf1;
+    if (a.fdec) a.fdec += (size_t)(bs.step - bs.fdec_step0) * a.fdec_stride;
+    tile_body<KIND, K, RAGGED>(a, it.strip, it.tile);
+}
+
+__global__ __launch_bounds__(256) void advance_kernel(BatchState* state, const uint32_t* entries,
+                                                      uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = entries[i];
+    BatchState& s = state[e & 0xFFFFFFu];
+    s.step += e >> 24;
+    s.cur ^= 1u;
+}
+
+hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
+                             const BatchState* state, const WorkItem* items, uint32_t n_items,
+                             hipStream_t s)
+{
+    if (!n_items) return hipSuccess;
+    const dim3 grid((n_items + 3) / 4);
+#define LGCA_BATCH_CASE(KD, KK)                                                                   \
+    if (kind == KD && k == KK) {                                                                \
+        if (ragged) hipLaunchKernelGGL((tile_batch_kernel<KD, KK, true>), grid, dim3(256), 0, s,  \
+                                       jobs, state, items, n_items);                            \
+        else hipLaunchKernelGGL((tile_batch_kernel<KD, KK, false>), grid, dim3(256), 0, s, jobs,  \
+                                state, items, n_items);                                         \
+        return hipGetLastError();                                                               \
+    }
+    LGCA_BATCH_CASE(kHPP, 1) LGCA_BATCH_CASE(kHPP, 2) LGCA_BATCH_CASE(kHPP, 4) LGCA_BATCH_CASE(kHPP, 8)
+    LGCA_BATCH_CASE(kFHP1, 1) LGCA_BATCH_CASE(kFHP1, 2) LGCA_BATCH_CASE(kFHP1, 4) LGCA_BATCH_CASE(kFHP1, 8)
+    LGCA_BATCH_CASE(kFHP2, 1) LGCA_BATCH_CASE(kFHP2, 2) LGCA_BATCH_CASE(kFHP2, 4) LGCA_BATCH_CASE(kFHP2, 8)
+#undef LGCA_BATCH_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_advance(BatchState* state, const uint32_t* entries, uint32_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(advance_kernel, dim3((n + 255) / 256), dim3(256), 0, s, state, entries, n);
+    return hipGetLastError();
 }
 
 uint32_t tile_halo(const Geometry& g, int k)

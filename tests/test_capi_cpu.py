@@ -2,6 +2,7 @@
 every entry point include/lgca.h declares, the ctypes signatures cover them,
 and the host-side mirror derives the reference's lattice sizes.  No compute
 call is made here (there may be no GPU)."""
+import ctypes as C
 import os
 import re
 
@@ -38,7 +39,7 @@ def test_library_exports_every_declared_symbol():
 def test_ctypes_signatures_cover_the_header():
     from lgca_amd import _capi
     names = set(declared_entry_points())
-    bound = set(_capi.SIGNATURES) | {"lgca_last_error"}
+    bound = set(_capi.SIGNATURES) | set(_capi.TEXT_ENTRY_POINTS)
     assert names == bound
 
 
@@ -90,3 +91,34 @@ def test_cpp_dropin_compiles_against_the_c_abi():
                         "-I", os.path.join(ROOT, "lgca_amd", "host"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_ctypes_structures_match_the_c_layout(tmp_path):
+    """sizeof / offsetof of every struct in include/lgca.h, from a C compiler,
+    equal the ctypes mirrors' (a mismatch would corrupt arguments silently)."""
+    import shutil
+    import subprocess
+    from lgca_amd import _capi
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    structs = {"lgca_info": _capi.Info, "lgca_job_desc": _capi.JobDesc,
+               "lgca_job_info": _capi.JobInfo, "lgca_batch_stats": _capi.BatchStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lgca.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    r = subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, field, value = line.split()
+        py = structs[cname]
+        want = C.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert int(value) == want, line

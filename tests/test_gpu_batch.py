@@ -1,0 +1,294 @@
+"""Batched ensembles and serving (lgca_batch_* of include/lgca.h), through the C ABI.
+
+The batch adds scheduling, not arithmetic: every job's final state bytes and
+FP32 coarse fields must equal, bit for bit, the oracle's run of the same job
+(set-up, hash fill, steps with the per-cell force, post-process), and a job's
+bits must not depend on what runs beside it, when it was admitted, or the
+launch shape the autotuner picked.  Large jobs, which the oracle would take
+minutes for, are checked against the single-lattice GPU path instead, itself
+pinned to the oracle by test_gpu_parity.py.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+Q = {0: 4, 1: 6, 2: 7, 3: 7}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def job_types(job):
+    """Cell types of a job as the reference set-up writes them."""
+    from lgca_amd import disc
+    from lgca_amd.batch import Job  # noqa: F401
+    n = job.dim_x * job.dim_y
+    if job.geometry == "types":
+        return np.ascontiguousarray(job.cell_types, dtype=np.uint8).reshape(-1)
+    t = np.zeros(n, dtype=np.uint8)
+    if job.geometry in ("pipe", "cylinder"):
+        g = t.reshape(job.dim_y, job.dim_x)
+        g[0, :] = 1
+        g[-1, :] = 1
+    if job.geometry == "cylinder":
+        t[disc(job.dim_x, job.dim_y, job.cyl_x, job.cyl_y, job.cyl_diameter)] = 1
+    return t
+
+
+def force_dict(job):
+    if job.force_dir is None or job.force_p <= 0:
+        return None
+    x0, x1, y0, y1 = job.force_region or (0, job.dim_x, 0, job.dim_y)
+    return dict(dir=job.force_dir, p=job.force_p, x0=x0, x1=x1, y0=y0, y1=y1)
+
+
+def oracle_job(job):
+    m = int(job.model)
+    types = job_types(job)
+    if job.state is not None:
+        st = np.ascontiguousarray(job.state, dtype=np.uint8).reshape(-1).copy()
+    else:
+        st = orc.init_random(m, job.dim_x, job.dim_y, types, job.seed, job.fill)
+    st = orc.run(m, job.dim_x, job.dim_y, types, st, job.seed, 0, job.steps, force=force_dict(job),
+                 nthreads=8)
+    _, _, md, mm = orc.post_process(m, job.dim_x, job.dim_y, job.coarse_radius, st)
+    return st, md, mm
+
+
+def single_lattice_job(job):
+    """The same job on the single-lattice GPU path (lgca_create .. lgca_step)."""
+    from lgca_amd import Lattice
+    lat = Lattice(int(job.model), dim_x=job.dim_x, dim_y=job.dim_y,
+                  coarse_graining_radius=job.coarse_radius, seed=job.seed)
+    try:
+        lat.set_cell_types(job_types(job))
+        if job.state is not None:
+            lat.set_state_u8(job.state)
+        else:
+            lat.init_random(job.fill)
+        f = force_dict(job)
+        if f:
+            lat.set_forcing(f["dir"], f["p"], f["x0"], f["x1"], f["y0"], f["y1"])
+        lat.step(job.steps)
+        st = lat.get_state_u8()
+        lat.copy_data_to_output_buffer()
+        lat.post_process(cells=False)
+        return st, lat.mean_density().copy(), lat.mean_momentum().copy()
+    finally:
+        lat.close()
+
+
+def same_bits(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                                 b.view(np.uint32) if b.dtype == np.float32 else b)
+
+
+def check_job(name, got, want):
+    st, md, mm = got
+    wst, wmd, wmm = want
+    assert np.array_equal(st, wst), f"{name}: state differs in {(st != wst).sum()} cells"
+    assert same_bits(md.reshape(-1), wmd.reshape(-1)), f"{name}: coarse density differs"
+    assert same_bits(mm.reshape(-1), wmm.reshape(-1)), f"{name}: coarse momentum differs"
+
+
+def mixed_jobs(rng):
+    """Small jobs over every model, geometry, width class, force and step
+    remainder the batch has to handle."""
+    from lgca_amd import Job
+    types70 = np.zeros(70 * 10, dtype=np.uint8)
+    g = types70.reshape(10, 70)
+    g[rng.random((10, 70)) < 0.08] = 1
+    g[rng.random((10, 70)) < 0.05] = 2
+    g[0, :] = 2
+    g[-1, :35] = 1
+    state64 = rng.integers(0, 1 << 7, 64 * 32, dtype=np.uint8)
+    return [
+        Job(0, 64, 48, 37, seed=1, fill=0.2, coarse_radius=8),
+        Job(0, 33, 20, 10, seed=2, fill=0.3, geometry="pipe", coarse_radius=2),
+        Job(0, 16, 6, 9, seed=3, fill=0.5),
+        Job(1, 96, 64, 25, seed=4, fill=0.25, geometry="cylinder", cyl_x=30, cyl_y=32,
+            cyl_diameter=20.0, force_dir="x", force_p=0.01, force_region=(0, 20, 0, 64),
+            coarse_radius=4),
+        Job(2, 200, 64, 41, seed=5, fill=0.3, geometry="pipe", force_dir="x", force_p=0.005,
+            coarse_radius=16),
+        Job(3, 128, 128, 64, seed=6, fill=0.2, geometry="cylinder", cyl_x=50, cyl_y=70,
+            cyl_diameter=33.5, force_dir="y", force_p=0.02, force_region=(10, 90, 5, 100),
+            coarse_radius=8),
+        Job(2, 70, 10, 13, seed=7, fill=0.4, geometry="types", cell_types=types70),
+        Job(2, 64, 32, 8, seed=8, state=state64, geometry="pipe", coarse_radius=2),
+        Job(3, 40, 20, 0, seed=9, fill=0.3),
+        Job(2, 2048, 64, 12, seed=10, fill=0.1, geometry="pipe", coarse_radius=16),
+        Job(1, 1000, 30, 17, seed=11, fill=0.2, force_dir="x", force_p=0.003, coarse_radius=3),
+        Job(2, 30, 2, 11, seed=12, fill=0.5),
+        Job(3, 100, 40, 33, seed=2 ** 40 + 13, fill=0.3, geometry="cylinder", cyl_x=-3, cyl_y=20,
+            cyl_diameter=17.0, force_dir="x", force_p=1.0, force_region=(60, 80, 0, 40)),
+    ]
+
+
+def run_batch(jobs, tuning=None, budget=8 * 10 ** 9):
+    from lgca_amd import Batch
+    with Batch(hbm_budget=budget) as b:
+        if tuning:
+            b.set_tuning(*tuning)
+        ids = [b.submit(j) for j in jobs]
+        b.wait(timeout=300)
+        out = [b.result(i) for i in ids]
+        stats = b.stats()
+        for i in ids:
+            assert b.info(i)["status"] == "done"
+            b.release(i)
+    return out, stats
+
+
+def test_mixed_jobs_match_oracle():
+    rng = np.random.default_rng(20)
+    jobs = mixed_jobs(rng)
+    say(f"[batch] {len(jobs)} mixed jobs")
+    got, stats = run_batch(jobs)
+    say(f"[batch] stats {stats}")
+    assert stats["jobs_done"] == len(jobs)
+    for i, job in enumerate(jobs):
+        check_job(f"job {i} {job.model}/{job.dim_x}x{job.dim_y}/{job.geometry}", got[i],
+                  oracle_job(job))
+
+
+@pytest.mark.parametrize("tuning", [(2, 256, 0), (4, 64, 0), (8, 100000, 0), (8, 0, 0),
+                                    (0, 0, 30000)])
+def test_results_do_not_depend_on_tuning_or_mix(tuning):
+    """Each job alone under a fixed launch shape (or with an active-cell cap
+    that serialises the mix) gives the bits it gives in the autotuned mix."""
+    rng = np.random.default_rng(20)
+    jobs = mixed_jobs(rng)
+    say(f"[batch] tuning {tuning}")
+    want, _ = run_batch(jobs)
+    got, _ = run_batch(jobs, tuning=tuning)
+    for i in range(len(jobs)):
+        check_job(f"job {i} tuning {tuning}", got[i], want[i])
+    alone, _ = run_batch([jobs[5]], tuning=tuning)
+    check_job("job 5 alone", alone[0], want[5])
+
+
+def test_budget_admission_and_release():
+    """With a budget that holds two jobs, the rest wait in order and are
+    admitted as finished jobs are released; a job larger than the budget is
+    refused at submit."""
+    from lgca_amd import Batch, Job, LgcaError, job_bytes
+    jobs = [Job(2, 256, 128, 24 + 4 * i, seed=100 + i, fill=0.3, geometry="pipe",
+                force_dir="x", force_p=0.002, coarse_radius=8) for i in range(6)]
+    per = max(job_bytes(j) for j in jobs)
+    budget = 2 * per + per // 2
+    say(f"[batch] budget {budget} bytes, {per} per job")
+    with Batch(hbm_budget=budget) as b:
+        with pytest.raises(LgcaError) as ei:
+            b.submit(Job(2, 4096, 2048, 4, fill=0.1))
+        assert ei.value.code == 5
+        ids = [b.submit(j) for j in jobs]
+        done, results = [], {}
+        t0 = time.monotonic()
+        while len(done) < len(ids):
+            assert time.monotonic() - t0 < 120, "jobs did not finish"
+            s = b.stats()
+            assert s["bytes_in_use"] <= budget
+            assert s["jobs_active"] <= 2
+            for i in b.poll():
+                results[i] = b.result(i)
+                b.release(i)
+                done.append(i)
+            time.sleep(0.002)
+        admits = [b.info(i)["admit_s"] for i in ids]
+        assert admits == sorted(admits), "admission is in submission order"
+    for k, i in enumerate(ids):
+        check_job(f"budget job {k}", results[i], single_lattice_job(jobs[k]))
+
+
+def test_continuous_batching_mid_run_arrivals():
+    """Jobs submitted while a long job runs are admitted at once, overlap with
+    it on the GPU and finish before it; every result equals the job run alone."""
+    from lgca_amd import Batch, Job
+    # ~10^4 rounds: far longer than submitting the short jobs takes
+    long_job = Job(3, 2048, 1024, 40000, seed=31, fill=0.25, geometry="cylinder", cyl_x=340,
+                   cyl_y=512, cyl_diameter=170.0, force_dir="x", force_p=0.002,
+                   force_region=(0, 64, 0, 1024), coarse_radius=16)
+    short = [Job(m, 128 + 64 * m, 64, 40 + 8 * m, seed=40 + m, fill=0.3,
+                 geometry="pipe" if m % 2 else "periodic", coarse_radius=4) for m in range(4)]
+    with Batch(hbm_budget=4 * 10 ** 9) as b:
+        lid = b.submit(long_job)
+        t0 = time.monotonic()
+        while b.info(lid)["status"] == "queued":
+            assert time.monotonic() - t0 < 30
+            time.sleep(0.0005)
+        sids = [b.submit(j) for j in short]
+        b.wait(timeout=300)
+        linfo = b.info(lid)
+        for i in sids:
+            si = b.info(i)
+            assert si["admit_s"] < linfo["done_s"]
+            assert si["done_s"] < linfo["done_s"], "short jobs finish while the long one runs"
+        got = {i: b.result(i) for i in [lid] + sids}
+    for k, j in enumerate(short):
+        check_job(f"short {k}", got[sids[k]], oracle_job(j))
+    check_job("long", got[lid], single_lattice_job(long_job))
+
+
+def test_c4_like_ensemble_matches_single_lattice_path():
+    """A slice of the 512-job ensemble: FHP-II jobs with long side 128..2048
+    (step 64), aspect 1..4, periodic / pipe / cylinder, fill 0.1-0.5, own seed;
+    steps cut to 60-200 so the comparison runs in seconds."""
+    from lgca_amd import Job
+    rng = np.random.default_rng(4)
+    jobs = []
+    for i in range(40):
+        L = int(rng.integers(2, 33)) * 64
+        a = float(rng.uniform(1.0, 4.0))
+        S = max(64, int(round(L / a / 64.0)) * 64)
+        geom = ["periodic", "pipe", "cylinder"][i % 3]
+        dx, dy = (L, S) if i % 2 == 0 else (S, L)
+        rad = int(rng.integers(8, max(9, min(dx, dy) // 8 + 1)))
+        jobs.append(Job(2, dx, dy, int(rng.integers(60, 201)), seed=1000 + i,
+                        fill=float(rng.uniform(0.1, 0.5)), geometry=geom, cyl_x=dx // 4,
+                        cyl_y=dy // 2, cyl_diameter=2.0 * rad, coarse_radius=16))
+    say(f"[batch] c4-like slice, {len(jobs)} jobs")
+    got, stats = run_batch(jobs)
+    say(f"[batch] stats {stats}")
+    for i, j in enumerate(jobs):
+        check_job(f"c4 job {i} {j.dim_x}x{j.dim_y} {j.geometry}", got[i], single_lattice_job(j))
+    # and a few small ones against the oracle directly
+    for i in (0, 1, 2):
+        if jobs[i].num_cells * jobs[i].steps <= 3e8:
+            check_job(f"c4 job {i} vs oracle", got[i], oracle_job(jobs[i]))
+
+
+def test_invalid_job_descriptions():
+    from lgca_amd import Batch, Job, LgcaError
+    bad_types = np.full(8 * 8, 3, dtype=np.uint8)
+    bad_state = np.full(8 * 8, 0x80, dtype=np.uint8)
+    cases = [
+        (Job(7, 8, 8, 1), 1),
+        (Job(2, 8, 7, 1), 1),
+        (Job(2, 8, 8, 1, coarse_radius=5), 1),
+        (Job(2, 8, 8, 1, fill=1.5), 1),
+        (Job(2, 8, 8, 1, force_dir="x", force_p=2.0), 1),
+        (Job(2, 8, 8, 1, geometry="types", cell_types=bad_types), 4),
+        (Job(2, 8, 8, 1, state=bad_state), 4),
+    ]
+    with Batch(hbm_budget=10 ** 9) as b:
+        for job, code in cases:
+            with pytest.raises(LgcaError) as ei:
+                b.submit(job)
+            assert ei.value.code == code, (job, ei.value)
+        jid = b.submit(Job(2, 8, 8, 3, fill=0.5))
+        b.wait(jid, timeout=60)
+        b.result(jid)
+        b.release(jid)
+        with pytest.raises(LgcaError):
+            b.result(jid)
+        with pytest.raises(LgcaError):
+            b.release(jid)
