@@ -164,6 +164,15 @@ LGCA_API lgca_status lgca_count_particles(lgca_lattice* h, uint64_t* count);
  * get_mean_velocity (src/omp_lattice.cpp:508-557).  Summed exactly. */
 LGCA_API lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2]);
 
+/* Checkpoint of the live state: model, sizes, seed, step, per-cell force
+ * settings, the count-mode force call counter, cell types and state bytes, in
+ * one file with a checksum.  A run resumed from it continues bit for bit as
+ * the saved run would have.  No counterpart in the reference, whose state
+ * lives only in the process (src/lgca_bitset.h:31-245). */
+LGCA_API lgca_status lgca_save_checkpoint(lgca_lattice* h, const char* path);
+/* Creates a handle on GPU `device` from a checkpoint file. */
+LGCA_API lgca_status lgca_load_checkpoint(lgca_lattice** out, const char* path, int device);
+
 /* ------------------------------------------------------------------------
  * Batched ensembles and serving.  No counterpart in the reference, which runs
  * one lattice per process: a job here is one run of that lattice, set up as

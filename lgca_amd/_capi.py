@@ -93,6 +93,8 @@ SIGNATURES = {
                                  C.POINTER(C.c_void_p)],
     "lgca_count_particles": [_h, C.POINTER(C.c_uint64)],
     "lgca_mean_velocity": [_h, _f32p],
+    "lgca_save_checkpoint": [_h, C.c_char_p],
+    "lgca_load_checkpoint": [C.POINTER(_h), C.c_char_p, C.c_int],
     "lgca_batch_create": [C.POINTER(_h), C.c_int, C.c_uint64],
     "lgca_batch_destroy": [_h],
     "lgca_batch_job_bytes": [C.POINTER(JobDesc), C.POINTER(C.c_uint64)],

@@ -209,6 +209,7 @@ struct Job {
     uint32_t cur = 0;        // buffer holding the state after the queued steps
     uint64_t window_end = 0; // force decisions exist for steps < window_end
     hipEvent_t done_ev = nullptr;
+    hipEvent_t last_ev = nullptr;  // end of the job's last round on the main stream
 };
 
 // First-fit allocator over a few large hipMalloc blocks.
@@ -292,7 +293,8 @@ struct Candidate {
 struct lgca_batch {
     int device = 0;
     uint64_t budget = 0;
-    hipStream_t stream = nullptr;       // worker: all GPU work of the jobs
+    hipStream_t stream = nullptr;       // worker: set-up and rounds
+    hipStream_t post_stream = nullptr;  // worker: post-processing of finished jobs
     hipStream_t copy_stream = nullptr;  // lgca_batch_result
     std::thread worker;
 
@@ -568,17 +570,23 @@ void admit(lgca_batch* b, Job* j)
 }
 
 // Coarse fields and state bytes of a finished job, then its completion event.
+// They run on the post-processing stream, after the job's last round on the
+// main stream: no later round touches the job's buffers, so the next rounds
+// overlap them.
 void finish(lgca_batch* b, Job* j)
 {
     const JobParams& p = j->p;
     char* m = j->mem;
     const uint32_t* st = (const uint32_t*)(m + j->L.buf) + (j->cur ? p.state_words : 0);
+    j->last_ev = take_event(b);
+    hipchk(hipEventRecord(j->last_ev, b->stream), "hipEventRecord");
+    hipchk(hipStreamWaitEvent(b->post_stream, j->last_ev, 0), "hipStreamWaitEvent");
     hipchk(launch_coarse(p.kind, st, p.g, j->d.coarse_radius, (float*)(m + j->L.md), (float*)(m + j->L.mm),
-                         (float*)(m + j->L.vel), b->stream),
+                         (float*)(m + j->L.vel), b->post_stream),
            "coarse");
-    hipchk(launch_unpack(st, (uint8_t*)(m + j->L.bytes), p.g, b->stream), "unpack");
+    hipchk(launch_unpack(st, (uint8_t*)(m + j->L.bytes), p.g, b->post_stream), "unpack");
     j->done_ev = take_event(b);
-    hipchk(hipEventRecord(j->done_ev, b->stream), "hipEventRecord");
+    hipchk(hipEventRecord(j->done_ev, b->post_stream), "hipEventRecord");
     b->finishing.push_back(j);
 }
 
@@ -604,7 +612,9 @@ void reap(lgca_batch* b, bool block)
         std::lock_guard<std::mutex> lk(b->mu);
         for (Job* j : done) {
             b->event_pool.push_back(j->done_ev);
+            b->event_pool.push_back(j->last_ev);
             j->done_ev = nullptr;
+            j->last_ev = nullptr;
             b->slot_job[j->slot] = nullptr;
             b->free_slots.push_back(j->slot);
             j->slot = -1;
@@ -1010,6 +1020,7 @@ extern "C" lgca_status lgca_batch_create(lgca_batch** out, int device, uint64_t 
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
     if ((e = hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
+    if ((e = hipStreamCreateWithFlags(&b->post_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
     if ((e = hipHostMalloc((void**)&b->staging, 2 * kStagingHalf, hipHostMallocDefault)) != hipSuccess)
         return fail(e, "hipHostMalloc(staging)");
     for (auto& ev : b->staging_ev)
@@ -1041,9 +1052,12 @@ extern "C" lgca_status lgca_batch_destroy(lgca_batch* b)
     if (b->worker.joinable()) b->worker.join();
     DevGuard dg(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->post_stream) (void)hipStreamSynchronize(b->post_stream);
     if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
-    for (auto& kv : b->jobs)
+    for (auto& kv : b->jobs) {
         if (kv.second->done_ev) (void)hipEventDestroy(kv.second->done_ev);
+        if (kv.second->last_ev) (void)hipEventDestroy(kv.second->last_ev);
+    }
     for (hipEvent_t e : b->inflight) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->event_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->tune_ev) (void)hipEventDestroy(e);
@@ -1058,6 +1072,7 @@ extern "C" lgca_status lgca_batch_destroy(lgca_batch* b)
         if (p) (void)hipHostFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
+    if (b->post_stream) (void)hipStreamDestroy(b->post_stream);
     delete b;
     return LGCA_OK;
 }

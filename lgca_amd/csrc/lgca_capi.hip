@@ -798,3 +798,135 @@ extern "C" lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2])
     out[1] = nf > 0 ? (float)(sy * unit_y / (840.0 * nf)) : 0.0f;
     return LGCA_OK;
 }
+
+// ------------------------------------------------------------ checkpoints
+// File layout (little-endian), all fixed-size fields first:
+//   "LGCACKP1"  8-byte magic
+//   u32 version (1), i32 model, u32 dim_x, u32 dim_y, u32 coarse_radius, u32 0,
+//   u64 seed, u64 step, u64 force_calls,
+//   i32 force_dir, u32 0, u64 force_thr, u32 fx0, fx1, fy0, fy1,
+//   u64 fnv1a64 of the payload,
+//   payload: num_cells cell-type bytes, then num_cells state bytes (bit d =
+//   direction d, the Bitset layout of src/lgca_bitset.h:229-231).
+namespace {
+
+struct CheckpointHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t model;
+    uint32_t dim_x, dim_y, coarse_radius, reserved0;
+    uint64_t seed, step, force_calls;
+    int32_t force_dir;
+    uint32_t reserved1;
+    uint64_t force_thr;
+    uint32_t fx0, fx1, fy0, fy1;
+    uint64_t checksum;
+};
+static_assert(sizeof(CheckpointHeader) == 96, "checkpoint header layout");
+constexpr char kCheckpointMagic[8] = {'L', 'G', 'C', 'A', 'C', 'K', 'P', '1'};
+
+uint64_t fnv1a64(const uint8_t* p, size_t n, uint64_t h = 0xcbf29ce484222325ull)
+{
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+
+}  // namespace
+
+extern "C" lgca_status lgca_save_checkpoint(lgca_lattice* h, const char* path)
+{
+    if (!h || !path) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    const size_t n = (size_t)h->dim_x * h->dim_y;
+    std::string buf(2 * n, '\0');
+    uint8_t* types = (uint8_t*)&buf[0];
+    uint8_t* state = types + n;
+    lgca_status st = ensure_bytes(h);
+    if (st) return st;
+    HIPCHK(h, launch_types_unpack(h->mask, h->bytes, h->g, h->stream));
+    HIPCHK(h, hipMemcpyAsync(types, h->bytes, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if ((st = download_state(h, state, hipMemcpyDeviceToHost))) return st;
+    CheckpointHeader hd{};
+    std::memcpy(hd.magic, kCheckpointMagic, 8);
+    hd.version = 1;
+    hd.model = h->model;
+    hd.dim_x = h->dim_x;
+    hd.dim_y = h->dim_y;
+    hd.coarse_radius = h->r;
+    hd.seed = h->seed;
+    hd.step = h->step;
+    hd.force_calls = h->force_calls;
+    hd.force_dir = h->force_dir;
+    hd.force_thr = h->force_thr;
+    hd.fx0 = h->fx0;
+    hd.fx1 = h->fx1;
+    hd.fy0 = h->fy0;
+    hd.fy1 = h->fy1;
+    hd.checksum = fnv1a64((const uint8_t*)buf.data(), buf.size());
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(h, LGCA_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+    const bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1 &&
+                    std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed) return fail(h, LGCA_ERR_INVALID_ARGUMENT, std::string("cannot write ") + path);
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_load_checkpoint(lgca_lattice** out, const char* path, int device)
+{
+    if (!out || !path) return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, "lgca_load_checkpoint: NULL argument");
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+    CheckpointHeader hd{};
+    std::string buf;
+    bool ok = std::fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, kCheckpointMagic, 8) == 0 &&
+              hd.version == 1 && hd.dim_x > 0 && hd.dim_y > 0 &&
+              (uint64_t)hd.dim_x * hd.dim_y < (1ull << 40) && hd.force_dir >= LGCA_FORCE_NONE &&
+              hd.force_dir <= LGCA_FORCE_Y && hd.force_thr <= (1ull << 32) && hd.fx1 <= hd.dim_x &&
+              hd.fy1 <= hd.dim_y;
+    if (ok) {
+        const size_t n = (size_t)hd.dim_x * hd.dim_y;
+        buf.resize(2 * n);
+        ok = std::fread(&buf[0], 1, buf.size(), f) == buf.size() && std::fgetc(f) == EOF &&
+             fnv1a64((const uint8_t*)buf.data(), buf.size()) == hd.checksum;
+    }
+    std::fclose(f);
+    if (!ok) return fail(nullptr, LGCA_ERR_INVALID_STATE, std::string("not a valid checkpoint: ") + path);
+    lgca_lattice* h = nullptr;
+    lgca_status st = lgca_create(&h, hd.model, hd.dim_x, hd.dim_y, hd.coarse_radius, device);
+    if (st) return st;
+    const size_t n = (size_t)hd.dim_x * hd.dim_y;
+    {
+        std::lock_guard<std::mutex> lk(h->mu);
+        DeviceGuard dg(h->device);
+        st = upload_types(h, (const uint8_t*)buf.data(), hipMemcpyHostToDevice);
+        if (!st) st = upload_state(h, (const uint8_t*)buf.data() + n, hipMemcpyHostToDevice);
+        if (!st) {
+            h->seed = hd.seed;
+            h->step = hd.step;
+            h->force_calls = hd.force_calls;
+            h->force_dir = hd.force_dir;
+            h->force_thr = hd.force_thr;
+            h->fx0 = hd.fx0;
+            h->fx1 = hd.fx1;
+            h->fy0 = hd.fy0;
+            h->fy1 = hd.fy1;
+            h->dec_count = 0;
+            const hipError_t e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) st = hip_fail(h, e, "hipStreamSynchronize");
+        }
+    }
+    if (st) {
+        g_create_error = h->err;
+        lgca_destroy(h);
+        return st;
+    }
+    *out = h;
+    return LGCA_OK;
+}

@@ -266,6 +266,13 @@ public:
         check(lgca_set_forcing(m_h, dir == 'x' ? LGCA_FORCE_X : (dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_NONE),
                                p, x0, x1, y0, y1));
     }
+    // Checkpoint of state, types, seed, step and forcing; resume with
+    // lgca_load_checkpoint (include/lgca.h).
+    void save_checkpoint(const std::string& path)
+    {
+        sync_types();
+        check(lgca_save_checkpoint(m_h, path.c_str()));
+    }
     void get_state_u8(uint8_t* out) { check(lgca_get_state_u8(m_h, out)); }
     void set_state_u8(const uint8_t* in) { check(lgca_set_state_u8(m_h, in)); }
     lgca_lattice* handle() { return m_h; }

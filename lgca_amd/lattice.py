@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -102,6 +103,31 @@ class Lattice:
         self._h = C.c_void_p()
         check(lib().lgca_create(C.byref(self._h), self.model, int(dim_x), int(dim_y), r,
                                 int(device)))
+        self._attach(r)
+        self.set_seed(seed)
+
+    @classmethod
+    def from_checkpoint(cls, path: str, device: int = 0) -> "Lattice":
+        """A lattice resumed from a file written by save_checkpoint: same
+        state, cell types, seed, step and forcing settings."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        check(lib().lgca_load_checkpoint(C.byref(self._h), os.fsencode(path), int(device)))
+        info = self.info()
+        self.model = int(info.model)
+        self.NUM_DIR = NUM_DIR[self.model]
+        self._p = dict(test_case=None, bf_dir=None, nu_s=f32(0), c_s=f32(1.0 / math.sqrt(2)),
+                       u=f32(0), Ma_s=f32(0), Re=f32(0))
+        self.m_bf_dir = None
+        self._attach(int(info.coarse_radius))
+        self.m_cell_type_cpu[:] = self.get_cell_types()
+        return self
+
+    def save_checkpoint(self, path: str) -> None:
+        self.copy_data_to_device()
+        self._check(lib().lgca_save_checkpoint(self._h, os.fsencode(path)))
+
+    def _attach(self, r: int) -> None:
         info = self.info()
         self.m_dim_x, self.m_dim_y = info.dim_x, info.dim_y
         self.m_num_cells = info.num_cells
@@ -115,7 +141,6 @@ class Lattice:
         self.m_cell_momentum_cpu = np.zeros(2 * self.m_num_cells, dtype=np.float32)
         self.m_mean_density_cpu = np.zeros(self.m_num_coarse_cells, dtype=np.float32)
         self.m_mean_momentum_cpu = np.zeros(2 * self.m_num_coarse_cells, dtype=np.float32)
-        self.set_seed(seed)
 
     # ---------------------------------------------------------------- handle
     def close(self) -> None:

@@ -1,10 +1,14 @@
 // viewer_loop.cpp -- replays the pipe viewer's use of the lattice
 // (apps/pipe/pipe_viewer.cpp:35-80 set-up, :100-184 ticks, :186-201 check)
 // through the C++ drop-in HIP_Lattice, and writes what happened for the
-// parity test (tests/test_native_dropin.py) to compare with the oracle.
+// parity test (tests/test_native_dropin.py) to compare with the oracle: the
+// tick log, the final state and coarse fields, the IoVti pair of .vti files
+// of the last tick (as the viewer's record button writes them, :162-167) and a
+// checkpoint.
 //
 //   viewer_loop <out_dir> <seed> <ticks>
 #include "hip_lattice.h"
+#include "io_vti.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -63,5 +67,12 @@ int main(int argc, char** argv)
     std::fwrite(lat.mean_density(), sizeof(Real), lat.num_coarse_cells(), f);
     std::fwrite(lat.mean_momentum(), sizeof(Real), 2 * lat.num_coarse_cells(), f);
     std::fclose(f);
+    IoVti<HIP_Lattice<Model::FHP_III>> vti(&lat, "Mean density");
+    vti.update();
+    if (!vti.write((size_t)ticks * PP_INTERVAL, out + "/")) {
+        std::fprintf(stderr, "could not write the .vti files\n");
+        return 1;
+    }
+    lat.save_checkpoint(out + "/checkpoint.bin");
     return n1 == n0 ? 0 : 1;
 }

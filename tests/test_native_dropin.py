@@ -57,3 +57,26 @@ def test_pipe_viewer_sequence_through_cpp_dropin(tmp_path):
     np.testing.assert_array_equal(mean[:md.size].view(np.uint32), md.view(np.uint32))
     np.testing.assert_array_equal(mean[md.size:].view(np.uint32), mm.view(np.uint32))
     assert calls > 0
+
+    # the IoVti files of the last tick hold the post-processed snapshot
+    from lgca_amd.checkpoint import read_checkpoint
+    from lgca_amd.io_vti import read_vti
+    cd, cm, md, mm = orc.post_process(3, dx, dy, 10, state)
+    cell = read_vti(str(tmp_path / f"cell_res_{5 * ticks}.vti"))
+    assert cell["extent"] == (0, dx, 0, dy, 0, 0) and cell["section"] == "CellData"
+    np.testing.assert_array_equal(cell["arrays"]["Cell density"].reshape(-1).view(np.uint32),
+                                  cd.view(np.uint32))
+    np.testing.assert_array_equal(cell["arrays"]["Cell momentum"].reshape(-1).view(np.uint32),
+                                  cm.view(np.uint32))
+    mean_v = read_vti(str(tmp_path / f"mean_res_{5 * ticks}.vti"))
+    assert mean_v["extent"] == (0, dx // 20 - 1, 0, dy // 20 - 1, 0, 0)
+    assert mean_v["scalars"] == "Mean density"
+    np.testing.assert_array_equal(mean_v["arrays"]["Mean density"].reshape(-1).view(np.uint32),
+                                  md.view(np.uint32))
+    np.testing.assert_array_equal(mean_v["arrays"]["Mean momentum"].reshape(-1).view(np.uint32),
+                                  mm.view(np.uint32))
+    # and the checkpoint the program saved last
+    ck = read_checkpoint(str(tmp_path / "checkpoint.bin"), verify=False)
+    assert (ck["seed"], ck["step"], ck["force_calls"]) == (seed, 5 * ticks, calls)
+    np.testing.assert_array_equal(ck["state"], state)
+    np.testing.assert_array_equal(ck["types"], types)
