@@ -66,28 +66,33 @@ __global__ __launch_bounds__(256) void geometry_kernel(uint8_t* __restrict__ typ
     types[(size_t)y * dim_x + x] = t;
 }
 
-// Force windows of many jobs: blockIdx.y picks the job, blocks stride over its
-// window's words.  The window starts at the job's current device step, which
-// becomes its fdec_step0.
+// Force windows of many jobs: blockIdx.y picks the job, threads stride over
+// its window's (group of four steps, row, word) triples.  The window starts at
+// the job's current device step, which becomes its fdec_step0.
 __global__ __launch_bounds__(256) void batch_draw_kernel(const BatchJob* __restrict__ jobs,
                                                          BatchState* state,
                                                          const uint32_t* __restrict__ slots,
                                                          uint32_t nsteps)
 {
+    __shared__ uint32_t Qt[32];
     const uint32_t slot = slots[blockIdx.y];
     const BatchJob& j = jobs[slot];
+    force_table(Qt, j.force_q);
     const uint32_t step0 = state[slot].step;
     const uint32_t rows = j.a.fy1 - j.a.fy0, fwn = j.a.fwn;
-    const uint64_t n = (uint64_t)nsteps * rows * fwn;
+    const uint32_t groups = ((step0 & 3u) + nsteps + 3u) / 4u;
+    const uint64_t n = (uint64_t)groups * rows * fwn;
     uint32_t* dec = const_cast<uint32_t*>(j.a.fdec);
     for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < n;
          idx += (uint64_t)gridDim.x * 256) {
         const uint32_t ww = (uint32_t)(idx % fwn);
         const uint32_t yy = (uint32_t)((idx / fwn) % rows);
-        const uint32_t k = (uint32_t)(idx / ((uint64_t)fwn * rows));
-        dec[idx] = force_draw_word(j.a.fw0 + ww, j.a.fy0 + yy, step0 + k, j.a.key0, j.a.key1,
-                                   j.force_thr, j.fx0, j.fx1, j.a.g.dim_x);
+        const uint32_t g = (uint32_t)(idx / ((uint64_t)fwn * rows));
+        const uint32_t w = j.a.fw0 + ww;
+        draw_group(dec + (size_t)yy * fwn + ww, (uint64_t)rows * fwn, nsteps, step0, g, w, j.a.fy0 + yy,
+                   j.a.key0, j.a.key1, Qt, force_columns(w, j.fx0, j.fx1, j.a.g.dim_x));
     }
+    // fdec_step0 is not the field the other blocks read
     if (blockIdx.x == 0 && threadIdx.x == 0) state[slot].fdec_step0 = step0;
 }
 
@@ -554,7 +559,7 @@ void admit(lgca_batch* b, Job* j)
         e.a.fw0 = p.fw0;
         e.a.fwn = p.fwn;
         e.a.force_dir = j->d.force_dir;
-        e.force_thr = p.force_thr;
+        e.force_q = force_q(p.force_thr);
         e.fx0 = p.fx0;
         e.fx1 = p.fx1;
     }
@@ -748,7 +753,7 @@ void draw_windows(lgca_batch* b, const std::vector<int>& ks)
     uint64_t most = 0;
     for (uint32_t s : slots) {
         const Job* j = b->slot_job[s];
-        most = std::max<uint64_t>(most, (uint64_t)kWindow * (j->p.fy1 - j->p.fy0) * j->p.fwn);
+        most = std::max<uint64_t>(most, (uint64_t)(kWindow / 4 + 1) * (j->p.fy1 - j->p.fy0) * j->p.fwn);
     }
     const uint32_t bx = (uint32_t)std::min<uint64_t>((most + 255) / 256, 128);
     // blockIdx.y covers at most 65535 jobs per launch

@@ -558,7 +558,7 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32
     fa.step0 = (uint32_t)h->step;
     fa.key0 = (uint32_t)h->seed;
     fa.key1 = (uint32_t)(h->seed >> 32);
-    fa.thr = h->force_thr;
+    fa.q = force_q(h->force_thr);
     fa.fx0 = h->fx0;
     fa.fx1 = h->fx1;
     fa.fy0 = h->fy0;

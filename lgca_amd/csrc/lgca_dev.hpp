@@ -170,26 +170,85 @@ __host__ __device__ __forceinline__ void apply_force(T s[7], T f, int dir)
     }
 }
 
-// Force decisions of one 32-cell word for one step (hash stream 1): bit b
-// fires when cell x = 32w + b lies in [fx0, fx1) and x < dim_x, and its draw
-// {x >> 2, y, step, 1}[x & 3] is below thr.
-__device__ __forceinline__ uint32_t force_draw_word(uint32_t w, uint32_t y, uint32_t step,
-                                                    uint32_t key0, uint32_t key1, uint64_t thr,
-                                                    uint32_t fx0, uint32_t fx1, uint32_t dim_x)
+// Per-cell force decisions of word w of row y at `step`: 32 independent
+// Bernoulli(thr / 2^32) bits by first-fire sampling.  u0 is the word's first
+// uniform, philox({w, y, step >> 2, 1})[step & 3] (one call serves four
+// steps); further uniforms, needed only after a fire, come from
+// philox({w, y, step, 5 + 4i}).  Qt[k] = Q_{k+1}, where Q_1 = q = 2^32 - thr
+// and Q_{k+1} = floor(Q_k q / 2^32) (force_table); the first of the next n
+// cells to fire is the smallest m < n with u + Q_{m+1} < 2^32, and there is
+// none when u + Q_n >= 2^32.
+__device__ __forceinline__ uint32_t force_word(uint32_t u0, uint32_t w, uint32_t y, uint32_t step,
+                                               uint32_t key0, uint32_t key1, const uint32_t* Qt)
 {
-    uint32_t word = 0;
-    for (uint32_t gi = 0; gi < 8; ++gi) {
-        const uint32_t x = 32 * w + 4 * gi;
-        if (x + 4 <= fx0 || x >= fx1 || x >= dim_x) continue;
-        const U4 r = philox4x32_10(x >> 2, y, step, 1u, key0, key1);
-        const uint32_t v[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-            const uint32_t xx = x + j;
-            if (xx >= fx0 && xx < fx1 && xx < dim_x && (uint64_t)v[j] < thr) word |= 1u << (4 * gi + j);
+    constexpr uint64_t kTwo32 = (uint64_t)1 << 32;
+    if ((uint64_t)u0 + Qt[31] >= kTwo32) return 0;  // no fire among all 32 cells
+    uint32_t word = 0, pos = 0, used = 0, u = u0;
+    U4 ext{0, 0, 0, 0};
+    for (;;) {
+        const uint32_t n = 32 - pos;
+        if ((uint64_t)u + Qt[n - 1] >= kTwo32) break;  // no fire among the next n cells
+        uint32_t lo = 0, hi = n - 1;                    // the condition is monotone in m
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)u + Qt[mid] < kTwo32) hi = mid;
+            else lo = mid + 1;
         }
+        word |= 1u << (pos + lo);
+        pos += lo + 1;
+        if (pos == 32) break;
+        if ((used & 3u) == 0) ext = philox4x32_10(w, y, step, 5u + 4u * (used >> 2), key0, key1);
+        const uint32_t i = used & 3u;
+        u = i == 0 ? ext.x : (i == 1 ? ext.y : (i == 2 ? ext.z : ext.w));
+        ++used;
     }
     return word;
+}
+
+// Fills Qt[0..31] = Q_1..Q_32 for q = 2^32 - thr (exact integer steps, the
+// same on every platform) and makes it visible to the whole workgroup.
+__device__ __forceinline__ void force_table(uint32_t* Qt, uint32_t q)
+{
+    if (threadIdx.x == 0) {
+        uint32_t Qk = q;
+        for (int k = 0; k < 32; ++k) {
+            Qt[k] = Qk;
+            Qk = __umulhi(Qk, q);
+        }
+    }
+    __syncthreads();
+}
+
+// Bits of word w inside the forced columns [fx0, fx1) and the lattice.
+__device__ __forceinline__ uint32_t force_columns(uint32_t w, uint32_t fx0, uint32_t fx1, uint32_t dim_x)
+{
+    const uint32_t x0 = 32 * w;
+    const uint32_t hi = min(min(fx1, dim_x), x0 + 32);
+    const uint32_t lo = max(fx0, x0);
+    if (lo >= hi) return 0;
+    const uint32_t nbits = hi - lo, sh = lo - x0;
+    return (nbits == 32 ? 0xFFFFFFFFu : ((1u << nbits) - 1u)) << sh;
+}
+
+// One thread's share of a force draw: the decision words of word w of row y
+// for the steps of group `group` (steps sharing (step & ~3), counted from
+// step0 & ~3) that fall in [step0, step0 + nsteps), stored at dec[k * plane]
+// for window index k; cols masks the forced columns of the word.
+__device__ __forceinline__ void draw_group(uint32_t* dec, uint64_t plane, uint32_t nsteps,
+                                           uint32_t step0, uint32_t group, uint32_t w, uint32_t y,
+                                           uint32_t key0, uint32_t key1, const uint32_t* Qt,
+                                           uint32_t cols)
+{
+    const uint32_t first = step0 & 3u;
+    const uint32_t tb = (step0 & ~3u) + 4u * group;  // first step of the group (mod 2^32)
+    const U4 r = philox4x32_10(w, y, tb >> 2, 1u, key0, key1);
+    const uint32_t u[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t k = 4u * group + i - first;  // window index of step tb + i
+        if (4u * group + i < first || k >= nsteps) continue;
+        dec[(size_t)k * plane] = cols ? (force_word(u[i], w, y, tb + i, key0, key1, Qt) & cols) : 0u;
+    }
 }
 
 }  // namespace lgca

@@ -215,25 +215,31 @@ hipError_t launch_step(int kind, const StepArgs& a, hipStream_t s)
 }
 
 // ------------------------------------------------------------ force draws
-// Decision word (step, y, w): bit b set when cell x = 32w + b lies in
-// [fx0, fx1) and philox({x>>2, y, step, 1})[x & 3] < thr.
+// Decision words of steps [step0, step0 + nsteps) of the forced rows and
+// words: bit b of word (step, y, w) forces cell 32w + b (see force_word).
+// A thread owns one word of one row for one group of four steps that share
+// the word's first Philox call; groups follow step >> 2 (steps mod 2^32).
 
 __global__ __launch_bounds__(256) void force_draw_kernel(ForceDrawArgs a)
 {
+    __shared__ uint32_t Qt[32];
+    force_table(Qt, a.q);
     const uint32_t rows = a.fy1 - a.fy0;
+    const uint32_t groups = ((a.step0 & 3u) + a.nsteps + 3u) / 4u;
     const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (uint64_t)a.nsteps * rows * a.fwn) return;
+    if (idx >= (uint64_t)groups * rows * a.fwn) return;
     const uint32_t ww = (uint32_t)(idx % a.fwn);
     const uint32_t yy = (uint32_t)((idx / a.fwn) % rows);
-    const uint32_t k = (uint32_t)(idx / ((uint64_t)a.fwn * rows));
-    const uint32_t word = force_draw_word(a.fw0 + ww, a.fy0 + yy, a.step0 + k, a.key0, a.key1, a.thr,
-                                          a.fx0, a.fx1, a.dim_x);
-    a.dec[((size_t)k * (a.fy1 - a.fy0) + yy) * a.fwn + ww] = word;
+    const uint32_t g = (uint32_t)(idx / ((uint64_t)a.fwn * rows));
+    const uint32_t w = a.fw0 + ww;
+    draw_group(a.dec + (size_t)yy * a.fwn + ww, (uint64_t)rows * a.fwn, a.nsteps, a.step0, g, w,
+               a.fy0 + yy, a.key0, a.key1, Qt, force_columns(w, a.fx0, a.fx1, a.dim_x));
 }
 
 hipError_t launch_force_draw(const ForceDrawArgs& a, hipStream_t s)
 {
-    const uint64_t n = (uint64_t)a.nsteps * (a.fy1 - a.fy0) * a.fwn;
+    const uint32_t groups = ((a.step0 & 3u) + a.nsteps + 3u) / 4u;
+    const uint64_t n = (uint64_t)groups * (a.fy1 - a.fy0) * a.fwn;
     hipLaunchKernelGGL(force_draw_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }

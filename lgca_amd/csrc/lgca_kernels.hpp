@@ -69,7 +69,7 @@ struct BatchJob {
     TileArgs a;            // geometry, masks, keys, force region and window base
     uint32_t* buf0;
     uint32_t* buf1;
-    uint64_t force_thr;    // per-cell force: fire when draw < force_thr
+    uint32_t force_q;      // per-cell force sampling parameter (force_q)
     uint32_t fx0, fx1;     // forced columns [fx0, fx1)
 };
 struct BatchState {
@@ -95,7 +95,7 @@ struct ForceDrawArgs {
     uint32_t nsteps;
     uint32_t step0;
     uint32_t key0, key1;
-    uint64_t thr;               // fire when draw < thr (thr <= 2^32)
+    uint32_t q;                 // first-fire sampling parameter, see force_q
     uint32_t fx0, fx1, fy0, fy1, fw0, fwn;
     uint32_t dim_x;
 };
@@ -108,6 +108,10 @@ inline uint64_t hash_threshold(double p)
     if (p >= 1.0) return (uint64_t)1 << 32;
     return (uint64_t)(p * 4294967296.0);  // p * 2^32 < 2^32: truncation is floor
 }
+
+// First-fire parameter of the per-cell force for threshold thr in [1, 2^32]:
+// q = 2^32 - thr (see force_word in lgca_dev.hpp).
+inline uint32_t force_q(uint64_t thr) { return (uint32_t)(((uint64_t)1 << 32) - thr); }
 
 hipError_t launch_pack(const uint8_t* state, uint32_t* planes, const Geometry& g,
                        uint32_t* err, hipStream_t s);

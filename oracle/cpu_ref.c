@@ -75,19 +75,62 @@ uint64_t oracle_threshold(double p)
 uint32_t oracle_chirality_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w)
 {
     uint32_t key[2], out[4];
-    const uint32_t ctr[4] = { w, y, (uint32_t)(step >> 2), 0u };
+    const uint32_t ctr[4] = { w, y, (uint32_t)step >> 2, 0u };
     seed_key(seed, key);
     oracle_philox4x32_10(ctr, key, out);
     return out[step & 3];
 }
 
-uint32_t oracle_force_draw(uint64_t seed, uint64_t step, uint32_t x, uint32_t y)
+/* Per-cell force decisions of one 32-cell word, by first-fire sampling: the
+ * gap to the next forced cell is drawn from its (truncated geometric) CDF,
+ * C[m] = P(first fire within m + 1 cells) * 2^32 = 2^32 - Q_{m+1}, where
+ * Q_k = floor(2^32 (1 - p)^k) is built by exact integer steps so every
+ * platform gets the same table. */
+static void force_cdf(uint64_t thr, uint64_t C[32])
 {
-    uint32_t key[2], out[4];
-    const uint32_t ctr[4] = { x >> 2, y, (uint32_t)step, 1u };
+    const uint64_t q = ((uint64_t)1 << 32) - thr;  /* thr >= 1, so q < 2^32 */
+    uint64_t Qk = q;                               /* Q_1 */
+    for (int m = 0; m < 32; ++m) {
+        C[m] = ((uint64_t)1 << 32) - Qk;           /* 2^32 - Q_{m+1} */
+        Qk = (Qk * q) >> 32;
+    }
+}
+
+static uint32_t force_word_cdf(const uint32_t key[2], const uint64_t C[32], uint64_t step,
+                               uint32_t y, uint32_t w)
+{
+    uint32_t prim[4], ext[4];
+    const uint32_t c0[4] = { w, y, (uint32_t)step >> 2, 1u };
+    oracle_philox4x32_10(c0, key, prim);
+    uint32_t u = prim[step & 3];
+    uint32_t word = 0;
+    unsigned pos = 0, used = 0;  /* used: extra uniforms taken so far */
+    for (;;) {
+        const unsigned n = 32 - pos;
+        if ((uint64_t)u >= C[n - 1]) break;       /* no fire among the next n cells */
+        unsigned m = 0;
+        while ((uint64_t)u >= C[m]) ++m;          /* first fire, m < n */
+        word |= 1u << (pos + m);
+        pos += m + 1;
+        if (pos == 32) break;
+        if ((used & 3) == 0) {
+            const uint32_t c1[4] = { w, y, (uint32_t)step, 5u + 4u * (used >> 2) };
+            oracle_philox4x32_10(c1, key, ext);
+        }
+        u = ext[used & 3];
+        ++used;
+    }
+    return word;
+}
+
+uint32_t oracle_force_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w, uint64_t thr)
+{
+    if (thr == 0) return 0;
+    uint32_t key[2];
+    uint64_t C[32];
     seed_key(seed, key);
-    oracle_philox4x32_10(ctr, key, out);
-    return out[x & 3];
+    force_cdf(thr, C);
+    return force_word_cdf(key, C, step, y, w);
 }
 
 uint32_t oracle_fill_draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t d)
@@ -325,13 +368,15 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
 {
     const int q = oracle_num_dir(model);
     uint32_t key[2];
+    uint64_t fcdf[32];
     seed_key(seed, key);
+    if (force_thr != 0) force_cdf(force_thr, fcdf);
     for (uint32_t y = y_begin; y < y_end; ++y) {
         uint32_t chir[4] = { 0, 0, 0, 0 };
         uint32_t chir_word = UINT32_MAX;
-        uint32_t fdraw[4] = { 0, 0, 0, 0 };
-        uint32_t fgroup = UINT32_MAX;
-        const int row_forced = force_dir != ORACLE_FORCE_NONE && y >= fy0 && y < fy1;
+        uint32_t fword = 0;
+        uint32_t fword_w = UINT32_MAX;
+        const int row_forced = force_dir != ORACLE_FORCE_NONE && force_thr != 0 && y >= fy0 && y < fy1;
         const int on_ns = (y == 0) || (y == dim_y - 1);
         /* Source row and x offset of every direction for this row, taken from
          * oracle_pull_source at x = 1 (the offset is the same for every x). */
@@ -361,7 +406,7 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
                 if (model != 0) {
                     const uint32_t w = x >> 5;
                     if (w != chir_word) {
-                        const uint32_t ctr[4] = { w, y, (uint32_t)(step >> 2), 0u };
+                        const uint32_t ctr[4] = { w, y, (uint32_t)step >> 2, 0u };
                         oracle_philox4x32_10(ctr, key, chir);
                         chir_word = w;
                     }
@@ -369,12 +414,11 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
                 }
                 o = lut->lut[p][s];
                 if (row_forced && x >= fx0 && x < fx1) {
-                    if ((x >> 2) != fgroup) {
-                        const uint32_t ctr[4] = { x >> 2, y, (uint32_t)step, 1u };
-                        oracle_philox4x32_10(ctr, key, fdraw);
-                        fgroup = x >> 2;
+                    if ((x >> 5) != fword_w) {
+                        fword = force_word_cdf(key, fcdf, step, y, x >> 5);
+                        fword_w = x >> 5;
                     }
-                    if ((uint64_t)fdraw[x & 3] < force_thr) o = apply_force(model, force_dir, o);
+                    if ((fword >> (x & 31)) & 1u) o = apply_force(model, force_dir, o);
                 }
             } else if (type == 1) {
                 o = oracle_bounce_back(model, s);

@@ -13,16 +13,26 @@
  *
  * Random choices (collision chirality, per-cell forcing, initial fill) come
  * from Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
- * as easy as 1, 2, 3", SC'11) keyed by the 64-bit lattice seed:
+ * as easy as 1, 2, 3", SC'11) keyed by the 64-bit lattice seed, with key =
+ * {seed & 0xffffffff, seed >> 32} and step taken mod 2^32:
  *   chirality word (y, w)   = philox(ctr = {w, y, step>>2, 0})[step & 3]
  *                             bit (x & 31) of word x>>5 is cell x's chirality
  *                             (one call serves a word for four steps)
- *   forcing draw (x, y)     = philox(ctr = {x>>2, y, step, 1})[x & 3]
- *   initial fill (x, y, d)  = philox(ctr = {x>>2, y, d,    2})[x & 3]
- * with key = {seed & 0xffffffff, seed >> 32} and step taken mod 2^32.  A draw
- * h fires when (uint64)h < threshold(p) = floor(p * 2^32) (clamped to
- * [0, 2^32]).  This replaces the reference's rand() calls
- * (src/lgca_bitset.h:284-288, src/utils.h:119-122, src/omp_lattice.cpp:269).
+ *   initial fill (x, y, d)  = philox(ctr = {x>>2, y, d, 2})[x & 3]; fires when
+ *                             (uint64)draw < thr
+ *   forcing word (y, w)     = 32 independent Bernoulli(thr / 2^32) bits drawn
+ *                             by first-fire sampling (oracle_force_word):
+ *                             uniform 0 is philox({w, y, step>>2, 1})[step & 3]
+ *                             (one call per word and four steps), uniform
+ *                             j >= 1 is philox({w, y, step, 5 + 4((j-1)>>2)})
+ *                             [(j-1) & 3]; with q = 2^32 - thr, Q_1 = q and
+ *                             Q_{k+1} = floor(Q_k q / 2^32), the first of the
+ *                             next n cells to fire is the smallest m < n with
+ *                             u < 2^32 - Q_{m+1} (none when u >= 2^32 - Q_n);
+ *                             bit (x & 31) forces cell x
+ * where thr = threshold(p) = floor(p * 2^32), clamped to [0, 2^32].  This
+ * replaces the reference's rand() calls (src/lgca_bitset.h:284-288,
+ * src/utils.h:119-122, src/omp_lattice.cpp:269).
  *
  * Model ids: 0 HPP, 1 FHP-I, 2 FHP-II, 3 FHP-III (src/lgca_common.h:49-54).
  */
@@ -41,7 +51,7 @@ enum { ORACLE_FORCE_NONE = 0, ORACLE_FORCE_X = 1, ORACLE_FORCE_Y = 2 };
 void     oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 uint64_t oracle_threshold(double p);
 uint32_t oracle_chirality_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w);
-uint32_t oracle_force_draw(uint64_t seed, uint64_t step, uint32_t x, uint32_t y);
+uint32_t oracle_force_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w, uint64_t thr);
 uint32_t oracle_fill_draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t d);
 
 int      oracle_num_dir(int model);

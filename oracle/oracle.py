@@ -44,8 +44,8 @@ def lib() -> C.CDLL:
         L.oracle_threshold.restype = u64
         L.oracle_chirality_word.argtypes = [u64, u64, u32, u32]
         L.oracle_chirality_word.restype = u32
-        L.oracle_force_draw.argtypes = [u64, u64, u32, u32]
-        L.oracle_force_draw.restype = u32
+        L.oracle_force_word.argtypes = [u64, u64, u32, u32, u64]
+        L.oracle_force_word.restype = u32
         L.oracle_fill_draw.argtypes = [u64, u32, u32, u32]
         L.oracle_fill_draw.restype = u32
         L.oracle_num_dir.argtypes = [i32]

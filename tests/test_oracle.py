@@ -63,11 +63,81 @@ def test_hash_streams_are_the_stated_philox_outputs(orc):
     key = [seed & 0xFFFFFFFF, seed >> 32]
     for (y, w, step) in [(0, 0, 0), (5, 7, 3), (8191, 1023, 4999), (3, 2, 2**33 + 6)]:
         assert L.oracle_chirality_word(seed, step, y, w) == \
-            orc.philox([w, y, (step >> 2) & 0xFFFFFFFF, 0], key)[step & 3]
-    for (x, y, step) in [(0, 0, 0), (63, 17, 12), (32767, 8191, 4999)]:
-        assert L.oracle_force_draw(seed, step, x, y) == orc.philox([x >> 2, y, step, 1], key)[x & 3]
+            orc.philox([w, y, (step & 0xFFFFFFFF) >> 2, 0], key)[step & 3]
     for (x, y, d) in [(0, 0, 0), (9, 3, 6)]:
         assert L.oracle_fill_draw(seed, x, y, d) == orc.philox([x >> 2, y, d, 2], key)[x & 3]
+
+
+# ------------------------------------------------- per-cell force contract
+
+def _leading_ones(u):
+    n = 0
+    while n < 32 and (u >> (31 - n)) & 1:
+        n += 1
+    return n
+
+
+def test_force_word_half_probability_closed_form(orc):
+    """At p = 1/2 the first-fire thresholds are 2^32 - 2^(31-m) exactly, so the
+    first forced cell among the next n is at offset (leading one-bits of the
+    uniform), none when that is >= n.  This pins the counter layout (which
+    Philox outputs, in which order) and the walk, independently of the
+    integer recurrence."""
+    L = orc.lib()
+    seed = 0xFEEDFACE12345678
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    thr = orc.threshold(0.5)
+    assert thr == 2 ** 31
+    for (y, w, step) in [(0, 0, 0), (1, 2, 3), (77, 5, 4999), (8191, 1023, 2 ** 33 + 7)]:
+        st = step & 0xFFFFFFFF
+        us = [orc.philox([w, y, st >> 2, 1], key)[step & 3]]
+        for j in range(32):
+            us.append(orc.philox([w, y, st, 5 + 4 * (j >> 2)], key)[j & 3])
+        want, pos, i = 0, 0, 0
+        while True:
+            m = _leading_ones(us[i])
+            i += 1
+            if m >= 32 - pos:
+                break
+            want |= 1 << (pos + m)
+            pos += m + 1
+            if pos == 32:
+                break
+        assert L.oracle_force_word(seed, step, y, w, thr) == want, (y, w, step)
+
+
+def test_force_word_edge_probabilities(orc):
+    L = orc.lib()
+    for w in range(50):
+        assert L.oracle_force_word(7, 11, 3, w, 0) == 0
+        assert L.oracle_force_word(7, 11, 3, w, 2 ** 32) == 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("p", [1e-3, 0.02, 0.3, 0.9])
+def test_force_word_is_iid_bernoulli(orc, p):
+    """Statistics over 20000 words: the fire rate is p, every bit position fires
+    at rate p, adjacent cells fire independently, and the count per word
+    follows Binomial(32, p).  A wrong threshold, an off-by-one in the walk or
+    a reused uniform shows up here."""
+    from math import comb, sqrt
+    L = orc.lib()
+    thr = orc.threshold(p)
+    words = np.array([L.oracle_force_word(12345, 1000 + (i % 7), i // 101, i % 101, thr)
+                      for i in range(20000)], dtype=np.uint64)
+    bits = ((words[:, None] >> np.arange(32, dtype=np.uint64)) & np.uint64(1)).astype(np.int64)
+    n = bits.size
+    rate = bits.mean()
+    assert abs(rate - p) < 5 * sqrt(p * (1 - p) / n)
+    per_pos = bits.mean(axis=0)
+    assert np.all(np.abs(per_pos - p) < 5 * sqrt(p * (1 - p) / bits.shape[0]))
+    pair = (bits[:, :-1] & bits[:, 1:]).mean()
+    assert abs(pair - p * p) < 5 * sqrt(p * p * (1 - p * p) / (bits.shape[0] * 31)) + 1e-4
+    counts = np.bincount(bits.sum(axis=1), minlength=33)
+    expect = np.array([comb(32, k) * p ** k * (1 - p) ** (32 - k) for k in range(33)]) * len(words)
+    keep = expect >= 5
+    chi2 = (((counts - expect) ** 2 / np.maximum(expect, 1e-12))[keep]).sum()
+    dof = max(1, int(keep.sum()) - 1)
+    assert chi2 < dof + 6 * sqrt(2 * dof), (chi2, dof)
 
 
 # ------------------------------------------------------- rules vs reference
