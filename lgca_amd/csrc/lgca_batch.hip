@@ -284,7 +284,7 @@ struct Arena {
 
 struct ClassRange {
     int kind, k;
-    bool ragged;
+    bool ragged, seg;
     uint32_t offset, count;
 };
 
@@ -640,18 +640,18 @@ int pick_k(int K, uint64_t left, uint64_t step)
     return 1;
 }
 
-uint32_t strips_of(const Geometry& g, int k)
+// Waves a job's round would have with one-row tiles: strips x rows, or rows /
+// (rows per wave) in segment mode.
+uint64_t job_wave_rows(const Geometry& g, int k)
 {
-    const uint32_t stride = 64u - 2u * tile_halo(g, k);
-    return (g.W + stride - 1) / stride;
+    if (tile_segments(g, k)) return ((uint64_t)g.dim_y + 64 / g.W - 1) / (64 / g.W);
+    return (uint64_t)tile_strips(g, k) * g.dim_y;
 }
 
-// Sum over running jobs of strips * rows: the waves a round would have with
-// one-row tiles.
 uint64_t wave_rows(const lgca_batch* b, int k)
 {
     uint64_t s = 0;
-    for (const Job* j : b->active) s += (uint64_t)strips_of(j->p.g, k) * j->d.dim_y;
+    for (const Job* j : b->active) s += job_wave_rows(j->p.g, k);
     return s;
 }
 
@@ -673,7 +673,7 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     for (size_t i = 0; i < n; ++i) b->sig.push_back(((uint64_t)b->active[i]->slot << 8) | (uint64_t)ks[i]);
     if (b->sig == b->last_sig) return;
     b->last_sig = b->sig;
-    // classes in a fixed order: kind, k, ragged
+    // classes in a fixed order: kind, k, ragged, segment mode
     std::map<uint64_t, std::vector<WorkItem>> cls;
     std::vector<uint32_t> entries;
     entries.reserve(n);
@@ -681,15 +681,23 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         const Job* j = b->active[i];
         const Geometry& g = j->p.g;
         const int k = ks[i];
-        const uint32_t halo = tile_halo(g, k);
-        const uint32_t strips = strips_of(g, k);
+        const bool seg = tile_segments(g, k);
+        const uint32_t halo = seg ? 0u : tile_halo(g, k);
         const uint32_t jth = std::min<uint32_t>(th, g.dim_y + (g.dim_y & 1u));
         const uint32_t tiles = (g.dim_y + jth - 1) / jth;
-        const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) | (g.nb != 32 ? 1u : 0u);
+        const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) |
+                             (g.nb != 32 ? 2u : 0u) | (seg ? 1u : 0u);
         std::vector<WorkItem>& v = cls[key];
-        for (uint32_t t = 0; t < tiles; ++t)
-            for (uint32_t s = 0; s < strips; ++s)
-                v.push_back(WorkItem{(uint32_t)j->slot, s, t, jth | (halo << 16)});
+        if (seg) {
+            const uint32_t S = 64 / g.W;
+            for (uint32_t t = 0; t < (tiles + S - 1) / S; ++t)
+                v.push_back(WorkItem{(uint32_t)j->slot, 0u, t, jth});
+        } else {
+            const uint32_t strips = tile_strips(g, k);
+            for (uint32_t t = 0; t < tiles; ++t)
+                for (uint32_t s = 0; s < strips; ++s)
+                    v.push_back(WorkItem{(uint32_t)j->slot, s, t, jth | (halo << 16)});
+        }
         entries.push_back((uint32_t)j->slot | ((uint32_t)k << 24));
     }
     size_t total = 0;
@@ -712,7 +720,8 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         ClassRange c;
         c.kind = (int)(kv.first >> 16);
         c.k = (int)((kv.first >> 8) & 0xFF);
-        c.ragged = (kv.first & 1u) != 0;
+        c.ragged = (kv.first & 2u) != 0;
+        c.seg = (kv.first & 1u) != 0;
         c.offset = (uint32_t)all.size();
         c.count = (uint32_t)kv.second.size();
         b->classes.push_back(c);
@@ -861,7 +870,7 @@ void run_round(lgca_batch* b)
         hipchk(hipEventRecord(b->tune_ev[2 * b->cand_i], b->stream), "hipEventRecord");
     uint64_t nl = 0;
     for (const ClassRange& c : b->classes) {
-        hipchk(launch_tile_batch(c.kind, c.k, c.ragged, b->d_jobs, b->d_state,
+        hipchk(launch_tile_batch(c.kind, c.k, c.ragged, c.seg, b->d_jobs, b->d_state,
                                  b->d_items[b->items_buf] + c.offset, c.count, b->stream),
                "tile_batch_kernel");
         ++nl;

@@ -577,10 +577,12 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32
 static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
 {
     if (h->tile_th) return h->tile_th + (h->tile_th & 1u);
-    const uint64_t strips = (h->g.W + 63 - 2 * tile_halo(h->g, k)) / (64 - 2 * tile_halo(h->g, k));
+    // rows x waves per row: strips per row, or 1 / (rows per wave) in segment mode
+    const uint64_t wave_rows = tile_segments(h->g, k) ? (uint64_t)h->dim_y / (64 / h->g.W)
+                                                      : (uint64_t)h->dim_y * tile_strips(h->g, k);
     // about 1.75 rounds of resident waves (four per SIMD on 1024 SIMDs); on the
     // 32768 x 8192 lattice this gives 20-row tiles, the fastest measured
-    uint64_t th = (uint64_t)h->dim_y * strips / 7168;
+    uint64_t th = wave_rows / 7168;
     if (th < 8) th = 8;
     if (th > 256) th = 256;
     if (th > h->dim_y) th = h->dim_y;
@@ -594,6 +596,8 @@ extern "C" lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint
     if (steps_per_pass != 0 && steps_per_pass != 1 && steps_per_pass != 2 && steps_per_pass != 4 &&
         steps_per_pass != 8)
         return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_tuning: steps_per_pass must be 0, 1, 2, 4 or 8");
+    if (tile_rows > 65534)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_tuning: tile_rows must be at most 65534");
     std::lock_guard<std::mutex> lk(h->mu);
     h->tile_k = steps_per_pass;
     h->tile_th = tile_rows;

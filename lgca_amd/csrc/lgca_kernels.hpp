@@ -58,7 +58,11 @@ struct TileArgs {
     uint32_t fy0, fy1, fw0, fwn;
     int32_t force_dir;
 };
-uint32_t tile_halo(const Geometry& g, int k);  // smallest valid TileArgs::halo
+uint32_t tile_halo(const Geometry& g, int k);    // smallest valid TileArgs::halo
+uint32_t tile_strips(const Geometry& g, int k);  // x-strips per row tile in strip mode
+// Segment mode: rows of W <= 64 words, S = 64 / W whole rows per wave (one per
+// row tile), used when it beats strips (S >= 2, or a row needing two strips).
+bool tile_segments(const Geometry& g, int k);
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s);
 
 // Batched ensembles.  BatchJob is a job's static descriptor (its TileArgs
@@ -79,13 +83,13 @@ struct BatchState {
     uint32_t pad;
 };
 struct WorkItem {
-    uint32_t job, strip, tile;
+    uint32_t job, strip, tile;  // segment mode: strip 0, tile = group of 64 / W row tiles
     uint32_t shape;        // rows per tile | halo << 16
 };
 // One class launch of a round: every item (all of one model, K and raggedness)
 // advances its job K steps reading its BatchState.  launch_advance then moves
 // each listed job (entry = slot | k << 24) k steps on and swaps its buffers.
-hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
+hipError_t launch_tile_batch(int kind, int k, bool ragged, bool seg, const BatchJob* jobs,
                              const BatchState* state, const WorkItem* items, uint32_t n_items,
                              hipStream_t s);
 hipError_t launch_advance(BatchState* state, const uint32_t* entries, uint32_t n, hipStream_t s);

@@ -47,21 +47,36 @@ struct LaneCtx {
     bool last;           // lane holds word W-1
     uint32_t nb;
     uint32_t vmask;      // valid bits of this lane's word
+    uint32_t lsrc, rsrc; // segment mode: byte address of the lanes holding words w-1, w+1
 };
 
+// Neighbouring words: the adjacent lane (DPP) when a wave holds consecutive
+// words of one row, or the lane holding word w -+ 1 of the same row in
+// segment mode, where the row wraps inside its segment (ds_bpermute).
+template <bool SEG>
+__device__ __forceinline__ uint32_t left_word(uint32_t v, const LaneCtx& c)
+{
+    return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.lsrc, (int)v) : from_left(v);
+}
+template <bool SEG>
+__device__ __forceinline__ uint32_t right_word(uint32_t v, const LaneCtx& c)
+{
+    return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.rsrc, (int)v) : from_right(v);
+}
+
 // out cell x = in cell x-1, cyclically over dim_x
-template <bool RAGGED>
+template <bool RAGGED, bool SEG>
 __device__ __forceinline__ uint32_t shift_m1(uint32_t v, const LaneCtx& c)
 {
-    uint32_t L = from_left(v);
+    uint32_t L = left_word<SEG>(v, c);
     if (RAGGED) L <<= c.first_shl;
     return __builtin_amdgcn_alignbit(v, L, 31);
 }
 // out cell x = in cell x+1, cyclically over dim_x
-template <bool RAGGED>
+template <bool RAGGED, bool SEG>
 __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 {
-    const uint32_t R = from_right(v);
+    const uint32_t R = right_word<SEG>(v, c);
     const uint32_t o = __builtin_amdgcn_alignbit(R, v, 1);
     if (!RAGGED) return o;
     return c.last ? ((v >> 1) | ((R & 1u) << (c.nb - 1u))) : o;
@@ -74,36 +89,59 @@ typedef const __attribute__((address_space(4))) int32_t* const_i32_ptr;
 // K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
 template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
 
-// One wave's work: x-strip `strip` of row tile `tile` of the lattice in `a`.
-template <int KIND, int K, bool RAGGED>
+// One wave's work.  Strip mode (SEG false): x-strip `strip` of row tile
+// `tile`.  Segment mode (SEG true, rows of W <= 64 words): the wave's lanes
+// form S = 64 / W segments of W lanes, each holding a whole row; segment s
+// walks row tile tile * S + s, and `strip` is unused.
+template <int KIND, int K, bool RAGGED, bool SEG>
 __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile)
 {
     constexpr int Q = num_dir_of_kind(KIND);
     const Geometry& g = a.g;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t stride = 64u - 2u * a.halo;
-    const uint32_t nstrips = (g.W + stride - 1) / stride;
-    if (strip >= nstrips) return;  // whole wave
     // tiles start on even rows (a.th is even), so the parity of the row a
     // level works on follows from the iteration index: see `odd_row` below
-    const uint32_t y0 = tile * a.th;
-    const uint32_t rows = min(a.th, g.dim_y - y0);
-
-    const int wabs = (int)(strip * stride) - (int)a.halo + (int)lane;
+    uint32_t y0, rows;  // per lane in segment mode
+    int wabs;
+    bool writer;
     LaneCtx c;
+    if (SEG) {
+        const uint32_t S = 64u / g.W;
+        const uint32_t ntiles = (g.dim_y + a.th - 1) / a.th;
+        if (tile * S >= ntiles) return;  // whole wave
+        const uint32_t seg = lane / g.W;  // S on the 64 - S * W idle lanes
+        const uint32_t base = seg * g.W;
+        const uint32_t mytile = tile * S + seg;
+        const bool live = seg < S && mytile < ntiles;
+        y0 = live ? mytile * a.th : 0u;
+        rows = live ? min(a.th, g.dim_y - y0) : 0u;
+        wabs = (int)(lane - base);  // < W, also on idle lanes
+        writer = live;
+        // (idle lanes may name a lane past 63; they never feed a live lane)
+        c.lsrc = 4u * ((wabs == 0 ? base + g.W - 1u : lane - 1u) & 63u);
+        c.rsrc = 4u * (((uint32_t)wabs + 1u == g.W ? base : lane + 1u) & 63u);
+    } else {
+        const uint32_t stride = 64u - 2u * a.halo;
+        const uint32_t nstrips = (g.W + stride - 1) / stride;
+        if (strip >= nstrips) return;  // whole wave
+        y0 = tile * a.th;
+        rows = min(a.th, g.dim_y - y0);
+        wabs = (int)(strip * stride) - (int)a.halo + (int)lane;
+        writer = lane >= a.halo && lane < 64u - a.halo && wabs < (int)g.W;
+        c.lsrc = c.rsrc = 0;
+    }
     c.w = (uint32_t)((wabs + (int)g.W) % (int)g.W);
     c.nb = g.nb;
     c.first_shl = (c.w == 0) ? 32u - g.nb : 0u;
     c.last = c.w + 1 == g.W;
     c.vmask = (c.last && g.nb < 32) ? ((1u << g.nb) - 1u) : 0xFFFFFFFFu;
-    const bool writer = lane >= a.halo && lane < 64u - a.halo && wabs < (int)g.W;
     uint32_t edge = 0;  // cells of this lane on the east / west columns
     if (c.w == 0) edge |= 1u;
     if (c.last) edge |= 1u << (g.nb - 1u);
 
     const uint32_t* src = a.src;
     uint32_t* dst = a.dst;
-    const uint32_t n_iter = rows + 2 * K;
+    const uint32_t n_iter = (SEG ? a.th : rows) + 2 * K;
     const int dy = (int)g.dim_y;
     auto wrap = [&](int r) -> uint32_t {
         if (r < 0) {
@@ -118,7 +156,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
 
     // Wave geometry for the uniform mask / force tests: does the wave (words
     // [wlo, wlo + 64) modulo W) touch words [lo, hi)?
-    const int wlo = (int)(strip * stride) - (int)a.halo;
+    const int wlo = (int)(strip * (64u - 2u * a.halo)) - (int)a.halo;  // strip mode only
     const int whi = wlo + 64;
     auto wave_hits = [&](int lo, int hi) -> bool {
         if (lo >= hi) return false;
@@ -136,9 +174,14 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     };
     const bool force_cols = a.fdec && wave_hits((int)a.fw0, (int)(a.fw0 + a.fwn));
     const bool force_mine = a.fdec && c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
-    // Decide once per tile whether the mask and force paths can be needed.
+    // Decide once per tile whether the mask and force paths can be needed
+    // (segment mode: whenever the lattice has masks / forcing; rows differ
+    // per segment).
     bool tile_solids = false, tile_force = false;
-    if (a.mask || force_cols) {
+    if (SEG) {
+        tile_solids = a.mask != nullptr;
+        tile_force = a.fdec != nullptr;
+    } else if (a.mask || force_cols) {
         for (uint32_t i = 0; i < n_iter; ++i) {
             const uint32_t r = wrap((int)(y0 + i) - K);
             if (a.mask && row_has_solids(r)) tile_solids = true;
@@ -176,7 +219,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     uint32_t nxt[7];
     uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
     {
-        if (tile_solids && row_has_solids(row_i)) {
+        if (tile_solids && (SEG || row_has_solids(row_i))) {
             nns = a.mask[((size_t)row_i * 2 + 0) * g.P + c.w];
             nsl = a.mask[((size_t)row_i * 2 + 1) * g.P + c.w];
         }
@@ -205,7 +248,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         if (i + 1 < n_iter) {
             const uint32_t rn = row_next;
             nns = nsl = 0;
-            if (tile_solids && row_has_solids(rn)) {
+            if (tile_solids && (SEG || row_has_solids(rn))) {
                 nns = a.mask[((size_t)rn * 2 + 0) * g.P + c.w];
                 nsl = a.mask[((size_t)rn * 2 + 1) * g.P + c.w];
             }
@@ -240,24 +283,24 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                 const uint32_t* A = winA[s - 1];
                 const uint32_t* B = winB[s - 1];
                 if (KIND == kHPP) {
-                    in[0] = shift_m1<RAGGED>(A[0], c);            // from (x-1, j)
+                    in[0] = shift_m1<RAGGED, SEG>(A[0], c);            // from (x-1, j)
                     in[1] = B[0];                         // from (x, j-1), plane 1 kept in B[0]
-                    in[2] = shift_p1<RAGGED>(A[2], c);    // from (x+1, j)
+                    in[2] = shift_p1<RAGGED, SEG>(A[2], c);    // from (x+1, j)
                     in[3] = cur[3];                       // from (x, j+1)
                 } else {
-                    in[0] = shift_m1<RAGGED>(A[0], c);
-                    in[3] = shift_p1<RAGGED>(A[3], c);
+                    in[0] = shift_m1<RAGGED, SEG>(A[0], c);
+                    in[3] = shift_p1<RAGGED, SEG>(A[3], c);
                     if (Q == 7) in[6] = A[6];
                     if (odd_row) {
                         in[1] = B[0];
-                        in[2] = shift_p1<RAGGED>(B[1], c);
-                        in[4] = shift_p1<RAGGED>(cur[4], c);
+                        in[2] = shift_p1<RAGGED, SEG>(B[1], c);
+                        in[4] = shift_p1<RAGGED, SEG>(cur[4], c);
                         in[5] = cur[5];
                     } else {
-                        in[1] = shift_m1<RAGGED>(B[0], c);
+                        in[1] = shift_m1<RAGGED, SEG>(B[0], c);
                         in[2] = B[1];
                         in[4] = cur[4];
-                        in[5] = shift_m1<RAGGED>(cur[5], c);
+                        in[5] = shift_m1<RAGGED, SEG>(cur[5], c);
                     }
                 }
                 uint32_t st[7];
@@ -289,9 +332,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                     const uint32_t ns = mns[s], sl = msl[s];
                     if (__builtin_amdgcn_ballot_w64((ns | sl) != 0) != 0)
                         apply_solids<KIND, uint32_t>(st, in, ns, sl, edge, j == 0 || j + 1 == g.dim_y);
-                    if (tile_force && j >= a.fy0 && j < a.fy1)
+                    // fcur is zero outside the forced rows
+                    if (tile_force && (SEG || (j >= a.fy0 && j < a.fy1)))
                         apply_force<KIND, uint32_t>(st, fcur[s] & ~(ns | sl), a.force_dir);
-                } else if (tile_force && j >= a.fy0 && j < a.fy1) {
+                } else if (tile_force && (SEG || (j >= a.fy0 && j < a.fy1))) {
                     apply_force<KIND, uint32_t>(st, fcur[s], a.force_dir);
                 }
 #pragma unroll
@@ -311,7 +355,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
             if (active) {
 #pragma unroll
                 for (int d = 0; d < Q; ++d) cur[d] = out[d];
-                if (s == K && writer) {
+                // segment mode: rows past a segment's own tile belong to the next
+                if (s == K && writer && (!SEG || i - (uint32_t)(2 * K) < rows)) {
                     uint32_t* q = dst + (size_t)j * Q * g.P;
 #pragma unroll
                     for (int d = 0; d < Q; ++d) q[(size_t)d * g.P + c.w] = cur[d];
@@ -370,20 +415,22 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     if (i < n_iter) iterate(T{}, P0{}, i);
 }
 
-template <int KIND, int K, bool RAGGED>
+template <int KIND, int K, bool RAGGED, bool SEG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_kernel(TileArgs a)
 {
-    tile_body<KIND, K, RAGGED>(a, __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6)),
-                               blockIdx.y);
+    // strip mode: grid (strips / 4, tiles); segment mode: grid (tile groups / 4)
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (SEG) tile_body<KIND, K, RAGGED, true>(a, 0, wave);
+    else tile_body<KIND, K, RAGGED, false>(a, wave, blockIdx.y);
 }
 
 // Batched form: every wave takes one work item (job, strip, tile, shape) and
 // runs that job's lattice.  A job's step, current buffer and force-window
 // origin live in device memory (advance_kernel moves them after each round),
 // so queued rounds need no host-to-device traffic.  One launch covers the
-// items of one (model, K, ragged) class.
-template <int KIND, int K, bool RAGGED>
+// items of one (model, K, ragged, segment mode) class.
+template <int KIND, int K, bool RAGGED, bool SEG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __restrict__ state,
                        const WorkItem* __restrict__ items, uint32_t n_items)
@@ -401,7 +448,7 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __re
     a.src = bs.cur ? jobs[it.job].buf1 : jobs[it.job].buf0;
     a.dst = bs.cur ? jobs[it.job].buf0 : jobs[it.job].buf1;
     if (a.fdec) a.fdec += (size_t)(bs.step - bs.fdec_step0) * a.fdec_stride;
-    tile_body<KIND, K, RAGGED>(a, it.strip, it.tile);
+    tile_body<KIND, K, RAGGED, SEG>(a, it.strip, it.tile);
 }
 
 __global__ __launch_bounds__(256) void advance_kernel(BatchState* state, const uint32_t* entries,
@@ -415,24 +462,31 @@ __global__ __launch_bounds__(256) void advance_kernel(BatchState* state, const u
     s.cur ^= 1u;
 }
 
-hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
+hipError_t launch_tile_batch(int kind, int k, bool ragged, bool seg, const BatchJob* jobs,
                              const BatchState* state, const WorkItem* items, uint32_t n_items,
                              hipStream_t s)
 {
     if (!n_items) return hipSuccess;
     const dim3 grid((n_items + 3) / 4);
-#define LGCA_BATCH_CASE(KD, KK)                                                                   \
-    if (kind == KD && k == KK) {                                                                \
-        if (ragged) hipLaunchKernelGGL((tile_batch_kernel<KD, KK, true>), grid, dim3(256), 0, s,  \
-                                       jobs, state, items, n_items);                            \
-        else hipLaunchKernelGGL((tile_batch_kernel<KD, KK, false>), grid, dim3(256), 0, s, jobs,  \
-                                state, items, n_items);                                         \
-        return hipGetLastError();                                                               \
+#define LGCA_BATCH_LAUNCH(KD, KK, RG, SG)                                                        \
+    hipLaunchKernelGGL((tile_batch_kernel<KD, KK, RG, SG>), grid, dim3(256), 0, s, jobs, state,  \
+                       items, n_items)
+#define LGCA_BATCH_CASE(KD, KK)                                                                  \
+    if (kind == KD && k == KK) {                                                               \
+        if (ragged) {                                                                          \
+            if (seg) LGCA_BATCH_LAUNCH(KD, KK, true, true);                                    \
+            else LGCA_BATCH_LAUNCH(KD, KK, true, false);                                       \
+        } else {                                                                               \
+            if (seg) LGCA_BATCH_LAUNCH(KD, KK, false, true);                                   \
+            else LGCA_BATCH_LAUNCH(KD, KK, false, false);                                      \
+        }                                                                                      \
+        return hipGetLastError();                                                              \
     }
     LGCA_BATCH_CASE(kHPP, 1) LGCA_BATCH_CASE(kHPP, 2) LGCA_BATCH_CASE(kHPP, 4) LGCA_BATCH_CASE(kHPP, 8)
     LGCA_BATCH_CASE(kFHP1, 1) LGCA_BATCH_CASE(kFHP1, 2) LGCA_BATCH_CASE(kFHP1, 4) LGCA_BATCH_CASE(kFHP1, 8)
     LGCA_BATCH_CASE(kFHP2, 1) LGCA_BATCH_CASE(kFHP2, 2) LGCA_BATCH_CASE(kFHP2, 4) LGCA_BATCH_CASE(kFHP2, 8)
 #undef LGCA_BATCH_CASE
+#undef LGCA_BATCH_LAUNCH
     return hipErrorInvalidValue;
 }
 
@@ -449,22 +503,50 @@ uint32_t tile_halo(const Geometry& g, int k)
     return (int)g.nb < k ? 2u : 1u;
 }
 
+uint32_t tile_strips(const Geometry& g, int k)
+{
+    const uint32_t stride = 64u - 2u * tile_halo(g, k);
+    return (g.W + stride - 1) / stride;
+}
+
+bool tile_segments(const Geometry& g, int k)
+{
+    return g.W <= 64 && (64 / g.W >= 2 || tile_strips(g, k) >= 2);
+}
+
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s)
 {
-    if (a.halo < tile_halo(a.g, k) || a.halo > 16) return hipErrorInvalidValue;
-    const uint32_t nstrips = (a.g.W + (64u - 2u * a.halo) - 1) / (64u - 2u * a.halo);
-    const dim3 grid((nstrips + 3) / 4, (a.g.dim_y + a.th - 1) / a.th);
+    const bool seg = tile_segments(a.g, k);
+    if (!seg && (a.halo < tile_halo(a.g, k) || a.halo > 16)) return hipErrorInvalidValue;
+    if (a.th < 2 || (a.th & 1u) || a.th > 65535) return hipErrorInvalidValue;
+    const uint32_t ntiles = (a.g.dim_y + a.th - 1) / a.th;
+    dim3 grid;
+    if (seg) {
+        const uint32_t groups = (ntiles + 64 / a.g.W - 1) / (64 / a.g.W);
+        grid = dim3((groups + 3) / 4, 1);
+    } else {
+        const uint32_t nstrips = (a.g.W + (64u - 2u * a.halo) - 1) / (64u - 2u * a.halo);
+        grid = dim3((nstrips + 3) / 4, ntiles);
+    }
     const bool ragged = a.g.nb != 32;
+#define LGCA_TILE_LAUNCH(KD, KK, RG, SG) \
+    hipLaunchKernelGGL((tile_kernel<KD, KK, RG, SG>), grid, dim3(256), 0, s, a)
 #define LGCA_TILE_CASE(KD, KK)                                                                 \
     if (kind == KD && k == KK) {                                                             \
-        if (ragged) hipLaunchKernelGGL((tile_kernel<KD, KK, true>), grid, dim3(256), 0, s, a);  \
-        else hipLaunchKernelGGL((tile_kernel<KD, KK, false>), grid, dim3(256), 0, s, a);        \
+        if (ragged) {                                                                        \
+            if (seg) LGCA_TILE_LAUNCH(KD, KK, true, true);                                   \
+            else LGCA_TILE_LAUNCH(KD, KK, true, false);                                      \
+        } else {                                                                             \
+            if (seg) LGCA_TILE_LAUNCH(KD, KK, false, true);                                  \
+            else LGCA_TILE_LAUNCH(KD, KK, false, false);                                     \
+        }                                                                                    \
         return hipGetLastError();                                                            \
     }
     LGCA_TILE_CASE(kHPP, 1) LGCA_TILE_CASE(kHPP, 2) LGCA_TILE_CASE(kHPP, 4) LGCA_TILE_CASE(kHPP, 8)
     LGCA_TILE_CASE(kFHP1, 1) LGCA_TILE_CASE(kFHP1, 2) LGCA_TILE_CASE(kFHP1, 4) LGCA_TILE_CASE(kFHP1, 8)
     LGCA_TILE_CASE(kFHP2, 1) LGCA_TILE_CASE(kFHP2, 2) LGCA_TILE_CASE(kFHP2, 4) LGCA_TILE_CASE(kFHP2, 8)
 #undef LGCA_TILE_CASE
+#undef LGCA_TILE_LAUNCH
     return hipErrorInvalidValue;
 }
 
