@@ -47,23 +47,91 @@ using namespace lgca;
 
 namespace lgca {
 
-// Cell types of the built-in geometries, one byte per cell: NO_SLIP rows 0 and
-// dim_y - 1 (walls), and a NO_SLIP disc given by its half-widths: cell (x, y)
-// is inside when |y - cy| < nh and |x - cx| <= halfw[|y - cy|].
-__global__ __launch_bounds__(256) void geometry_kernel(uint8_t* __restrict__ types, uint32_t dim_x,
-                                                       uint32_t dim_y, int walls, int cx, int cy,
-                                                       const int32_t* __restrict__ halfw, int nh)
+// One admitted job: its job-table entry and how to set it up on the device.
+struct JobInstall {
+    BatchJob job;
+    uint32_t slot;
+    int32_t geometry;      // LGCA_GEOM_*
+    int32_t cx, cy;        // disc centre
+    uint32_t halfw_off;    // disc half-widths at halfw[halfw_off .. + nh)
+    uint32_t nh;
+    uint32_t generated;    // 1: cell types from the geometry and state from the hash fill
+    uint32_t pad;
+    uint64_t fill_thr;
+};
+
+__global__ __launch_bounds__(256) void install_kernel(BatchJob* jobs, BatchState* state,
+                                                      const JobInstall* __restrict__ rec, uint32_t n)
 {
-    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t y = blockIdx.y;
-    if (x >= dim_x) return;
-    uint8_t t = LGCA_FLUID;
-    if (walls && (y == 0 || y == dim_y - 1)) t = LGCA_SOLID_NO_SLIP;
-    if (halfw) {
-        const int dy = abs((int)y - cy);
-        if (dy < nh && abs((int)x - cx) <= halfw[dy]) t = LGCA_SOLID_NO_SLIP;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    jobs[rec[i].slot] = rec[i].job;
+    state[rec[i].slot] = BatchState{0u, 0u, 0u, 0u};
+}
+
+// Bits of word w whose cells x = 32w + b lie in [xa, xb].
+__device__ __forceinline__ uint32_t bits_in_range(uint32_t w, int64_t xa, int64_t xb)
+{
+    const int64_t lo = max(xa - 32 * (int64_t)w, (int64_t)0);
+    const int64_t hi = min(xb - 32 * (int64_t)w, (int64_t)31);
+    if (lo > hi) return 0u;
+    const uint32_t n = (uint32_t)(hi - lo + 1);
+    return (n == 32 ? 0xFFFFFFFFu : ((1u << n) - 1u)) << (uint32_t)lo;
+}
+
+// Set-up of the generated jobs of one admission, blockIdx.y = record: NO_SLIP
+// walls on rows 0 and dim_y - 1 (pipe, cylinder) and the disc |y - cy| < nh,
+// |x - cx| <= halfw[|y - cy|] (cylinder) as mask planes with their per-row
+// word ranges, the hash fill of the FLUID cells in buffer 0 and zeros in
+// buffer 1 (padding words included).
+__global__ __launch_bounds__(256) void setup_kernel(const JobInstall* __restrict__ rec,
+                                                    const int32_t* __restrict__ halfw_all)
+{
+    const JobInstall& r = rec[blockIdx.y];
+    if (!r.generated) return;
+    const Geometry g = r.job.a.g;
+    uint32_t* mask = const_cast<uint32_t*>(r.job.a.mask);
+    int32_t* rr = const_cast<int32_t*>(r.job.a.row_range);
+    const int32_t* hw = halfw_all + r.halfw_off;
+    const uint64_t n = (uint64_t)g.dim_y * g.P;
+    for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (uint64_t)gridDim.x * 256) {
+        const uint32_t y = (uint32_t)(idx / g.P), w = (uint32_t)(idx % g.P);
+        const uint32_t valid = w + 1 < g.W ? 0xFFFFFFFFu
+                             : (w + 1 == g.W ? (g.nb == 32 ? 0xFFFFFFFFu : (1u << g.nb) - 1u) : 0u);
+        uint32_t ns = 0;
+        int32_t lo = 0x7FFFFFFF, hi = 0;
+        if (r.geometry != LGCA_GEOM_PERIODIC && (y == 0 || y + 1 == g.dim_y)) {
+            ns = valid;
+            lo = 0;
+            hi = (int32_t)g.W;
+        }
+        if (r.geometry == LGCA_GEOM_CYLINDER) {
+            const int64_t dy = abs((int64_t)y - r.cy);
+            if (dy < (int64_t)r.nh) {
+                const int64_t xa = max((int64_t)r.cx - hw[dy], (int64_t)0);
+                const int64_t xb = min((int64_t)r.cx + hw[dy], (int64_t)g.dim_x - 1);
+                if (xa <= xb) {
+                    ns |= bits_in_range(w, xa, xb) & valid;
+                    lo = min(lo, (int32_t)(xa >> 5));
+                    hi = max(hi, (int32_t)(xb >> 5) + 1);
+                }
+            }
+        }
+        if (mask) {
+            mask[((size_t)y * 2 + 0) * g.P + w] = ns;
+            mask[((size_t)y * 2 + 1) * g.P + w] = 0u;
+            if (w == 0) {
+                rr[y] = lo;
+                rr[g.dim_y + y] = hi;
+            }
+        }
+        const uint32_t fluid = valid & ~ns;
+        for (uint32_t d = 0; d < g.Q; ++d) {
+            const size_t at = ((size_t)y * g.Q + d) * g.P + w;
+            r.job.buf0[at] = fluid ? (fill_word(w, y, d, g.dim_x, r.job.a.key0, r.job.a.key1, r.fill_thr) & fluid) : 0u;
+            r.job.buf1[at] = 0u;
+        }
     }
-    types[(size_t)y * dim_x + x] = t;
 }
 
 // Force windows of many jobs: blockIdx.y picks the job, threads stride over
@@ -338,6 +406,8 @@ struct lgca_batch {
     int items_buf = 0;
     uint32_t* d_draw = nullptr;
     size_t draw_cap = 0;
+    char* d_setup = nullptr;            // admission records and disc tables
+    size_t setup_cap = 0;
     uint32_t* d_err = nullptr;          // sink for the codec kernels' flags
     char* staging = nullptr;            // pinned, two halves
     size_t staging_used = 0;
@@ -500,78 +570,136 @@ void grow_tables(lgca_batch* b, uint32_t need)
     b->cap = cap;
 }
 
-// Sets up an admitted job on the device: zeroed buffers, masks, initial state,
-// its job-table entry and device state.
-void admit(lgca_batch* b, Job* j)
+// Grows a worker-owned device buffer; the stream is drained first because
+// queued work may still read the old one.
+void grow_buffer(lgca_batch* b, char** buf, size_t* cap, size_t need, const char* what)
 {
-    if (b->free_slots.empty()) grow_tables(b, b->cap + 1);
-    j->slot = b->free_slots.back();
-    b->free_slots.pop_back();
-    b->slot_job[j->slot] = j;
-    const JobParams& p = j->p;
-    const Geometry& g = p.g;
-    char* m = j->mem;
-    uint32_t* buf0 = (uint32_t*)(m + j->L.buf);
-    uint32_t* buf1 = buf0 + p.state_words;
-    uint32_t* mask = p.has_mask ? (uint32_t*)(m + j->L.mask) : nullptr;
-    int32_t* rr = p.has_mask ? (int32_t*)(m + j->L.row_range) : nullptr;
-    uint8_t* bytes = (uint8_t*)(m + j->L.bytes);
-    hipchk(launch_fill_u32(buf0, 0u, 2 * (uint64_t)p.state_words, b->stream), "fill");
-    if (p.has_mask) {
-        if (j->d.geometry == LGCA_GEOM_TYPES) {
-            upload(b, bytes, j->types_in.data(), j->types_in.size());
-        } else {
-            const int32_t* hw = nullptr;
-            if (!j->halfw.empty()) {
-                int32_t* dhw = (int32_t*)(m + j->L.halfw);
-                upload(b, dhw, j->halfw.data(), j->halfw.size() * 4);
-                hw = dhw;
-            }
-            hipLaunchKernelGGL(geometry_kernel, dim3((j->d.dim_x + 255) / 256, j->d.dim_y), dim3(256), 0,
-                               b->stream, bytes, j->d.dim_x, j->d.dim_y, 1, j->d.cyl_x, j->d.cyl_y, hw,
-                               (int)j->halfw.size());
-            hipchk(hipGetLastError(), "geometry_kernel");
+    if (need <= *cap) return;
+    hipchk(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    const size_t c = std::max(need + need / 2, (size_t)1 << 20);
+    hipchk(hipMalloc((void**)buf, c), what);
+    *cap = c;
+}
+
+// Sets up newly admitted jobs: one install launch writes their job-table
+// entries and zero device states, one set-up launch builds the masks and hash
+// fill of every job with a built-in geometry; jobs with given cell types or
+// state take the upload and codec path.
+void admit_all(lgca_batch* b, const std::vector<Job*>& jobs)
+{
+    if (jobs.empty()) return;
+    if (b->free_slots.size() < jobs.size()) grow_tables(b, b->cap + (uint32_t)jobs.size());
+    std::vector<JobInstall> rec(jobs.size());
+    std::vector<int32_t> halfw;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        Job* j = jobs[i];
+        j->slot = b->free_slots.back();
+        b->free_slots.pop_back();
+        b->slot_job[j->slot] = j;
+        const JobParams& p = j->p;
+        char* m = j->mem;
+        JobInstall& r = rec[i];
+        std::memset(&r, 0, sizeof r);
+        BatchJob& e = r.job;
+        e.a.g = p.g;
+        e.a.mask = p.has_mask ? (const uint32_t*)(m + j->L.mask) : nullptr;
+        e.a.row_range = p.has_mask ? (const int32_t*)(m + j->L.row_range) : nullptr;
+        e.a.key0 = (uint32_t)j->d.seed;
+        e.a.key1 = (uint32_t)(j->d.seed >> 32);
+        if (p.forced) {
+            e.a.fdec = (const uint32_t*)(m + j->L.fdec);
+            e.a.fdec_stride = (uint64_t)(p.fy1 - p.fy0) * p.fwn;
+            e.a.fy0 = p.fy0;
+            e.a.fy1 = p.fy1;
+            e.a.fw0 = p.fw0;
+            e.a.fwn = p.fwn;
+            e.a.force_dir = j->d.force_dir;
+            e.force_q = force_q(p.force_thr);
+            e.fx0 = p.fx0;
+            e.fx1 = p.fx1;
         }
-        hipchk(launch_fill_u32((uint32_t*)rr, 0x7FFFFFFFu, j->d.dim_y, b->stream), "fill");
-        hipchk(launch_fill_u32((uint32_t*)rr + j->d.dim_y, 0u, j->d.dim_y, b->stream), "fill");
-        hipchk(launch_types(bytes, mask, rr, g, b->d_err, b->stream), "types");
+        e.buf0 = (uint32_t*)(m + j->L.buf);
+        e.buf1 = e.buf0 + p.state_words;
+        r.slot = (uint32_t)j->slot;
+        r.geometry = j->d.geometry;
+        r.cx = j->d.cyl_x;
+        r.cy = j->d.cyl_y;
+        r.halfw_off = (uint32_t)halfw.size();
+        r.nh = (uint32_t)j->halfw.size();
+        halfw.insert(halfw.end(), j->halfw.begin(), j->halfw.end());
+        r.generated = j->d.geometry != LGCA_GEOM_TYPES && j->state_in.empty();
+        r.fill_thr = hash_threshold(j->d.fill);
+        b->h_jobs[j->slot] = e;
+        j->step = 0;
+        j->cur = 0;
+        j->window_end = 0;
     }
-    if (!j->state_in.empty()) {
-        upload(b, bytes, j->state_in.data(), j->state_in.size());
-        hipchk(launch_pack(bytes, buf0, g, b->d_err, b->stream), "pack");
-    } else {
-        hipchk(launch_init_fill(buf0, mask, nullptr, g, (uint32_t)j->d.seed, (uint32_t)(j->d.seed >> 32),
-                                hash_threshold(j->d.fill), b->stream),
-               "init_fill");
+    const size_t rec_bytes = align_up(rec.size() * sizeof(JobInstall));
+    grow_buffer(b, &b->d_setup, &b->setup_cap, rec_bytes + halfw.size() * 4 + 16, "hipMalloc(set-up)");
+    const JobInstall* d_rec = (const JobInstall*)b->d_setup;
+    const int32_t* d_halfw = (const int32_t*)(b->d_setup + rec_bytes);
+    upload(b, b->d_setup, rec.data(), rec.size() * sizeof(JobInstall));
+    if (!halfw.empty()) upload(b, b->d_setup + rec_bytes, halfw.data(), halfw.size() * 4);
+    const uint32_t n = (uint32_t)rec.size();
+    hipLaunchKernelGGL(install_kernel, dim3((n + 255) / 256), dim3(256), 0, b->stream, b->d_jobs, b->d_state,
+                       d_rec, n);
+    hipchk(hipGetLastError(), "install_kernel");
+    uint64_t most = 0;
+    for (const Job* j : jobs) most = std::max<uint64_t>(most, (uint64_t)j->d.dim_y * j->p.g.P);
+    const uint32_t bx = (uint32_t)std::min<uint64_t>((most + 255) / 256, 256);
+    for (uint32_t o = 0; o < n; o += 65535) {
+        const uint32_t ny = std::min<uint32_t>(65535, n - o);
+        hipLaunchKernelGGL(setup_kernel, dim3(bx, ny), dim3(256), 0, b->stream, d_rec + o, d_halfw);
+        hipchk(hipGetLastError(), "setup_kernel");
     }
-    BatchJob& e = b->h_jobs[j->slot];
-    std::memset(&e, 0, sizeof e);
-    e.a.g = g;
-    e.a.mask = mask;
-    e.a.row_range = rr;
-    e.a.key0 = (uint32_t)j->d.seed;
-    e.a.key1 = (uint32_t)(j->d.seed >> 32);
-    if (p.forced) {
-        e.a.fdec = (const uint32_t*)(m + j->L.fdec);
-        e.a.fdec_stride = (uint64_t)(p.fy1 - p.fy0) * p.fwn;
-        e.a.fy0 = p.fy0;
-        e.a.fy1 = p.fy1;
-        e.a.fw0 = p.fw0;
-        e.a.fwn = p.fwn;
-        e.a.force_dir = j->d.force_dir;
-        e.force_q = force_q(p.force_thr);
-        e.fx0 = p.fx0;
-        e.fx1 = p.fx1;
+    // given cell types or state
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (rec[i].generated) continue;
+        Job* j = jobs[i];
+        const JobParams& p = j->p;
+        char* m = j->mem;
+        uint32_t* buf0 = (uint32_t*)(m + j->L.buf);
+        uint32_t* mask = p.has_mask ? (uint32_t*)(m + j->L.mask) : nullptr;
+        int32_t* rr = p.has_mask ? (int32_t*)(m + j->L.row_range) : nullptr;
+        uint8_t* bytes = (uint8_t*)(m + j->L.bytes);
+        hipchk(launch_fill_u32(buf0, 0u, 2 * (uint64_t)p.state_words, b->stream), "fill");
+        if (p.has_mask) {
+            if (j->d.geometry == LGCA_GEOM_TYPES) {
+                upload(b, bytes, j->types_in.data(), j->types_in.size());
+            } else {
+                // built-in geometry with a given state: the set-up kernel's
+                // masks, made here from bytes as for given types
+                std::vector<uint8_t> t((size_t)j->d.dim_x * j->d.dim_y, (uint8_t)LGCA_FLUID);
+                for (uint32_t x = 0; x < j->d.dim_x; ++x) {
+                    t[x] = LGCA_SOLID_NO_SLIP;
+                    t[(size_t)(j->d.dim_y - 1) * j->d.dim_x + x] = LGCA_SOLID_NO_SLIP;
+                }
+                for (size_t dy = 0; dy < j->halfw.size(); ++dy)
+                    for (int sgn = -1; sgn <= 1; sgn += 2) {
+                        const int64_t y = (int64_t)j->d.cyl_y + sgn * (int64_t)dy;
+                        if (y < 0 || y >= (int64_t)j->d.dim_y || (sgn == 1 && dy == 0)) continue;
+                        const int64_t xa = std::max<int64_t>((int64_t)j->d.cyl_x - j->halfw[dy], 0);
+                        const int64_t xb = std::min<int64_t>((int64_t)j->d.cyl_x + j->halfw[dy], (int64_t)j->d.dim_x - 1);
+                        for (int64_t x = xa; x <= xb; ++x) t[(size_t)y * j->d.dim_x + x] = LGCA_SOLID_NO_SLIP;
+                    }
+                upload(b, bytes, t.data(), t.size());
+            }
+            hipchk(launch_fill_u32((uint32_t*)rr, 0x7FFFFFFFu, j->d.dim_y, b->stream), "fill");
+            hipchk(launch_fill_u32((uint32_t*)rr + j->d.dim_y, 0u, j->d.dim_y, b->stream), "fill");
+            hipchk(launch_types(bytes, mask, rr, p.g, b->d_err, b->stream), "types");
+        }
+        if (!j->state_in.empty()) {
+            upload(b, bytes, j->state_in.data(), j->state_in.size());
+            hipchk(launch_pack(bytes, buf0, p.g, b->d_err, b->stream), "pack");
+        } else {
+            hipchk(launch_init_fill(buf0, mask, nullptr, p.g, (uint32_t)j->d.seed, (uint32_t)(j->d.seed >> 32),
+                                    hash_threshold(j->d.fill), b->stream),
+                   "init_fill");
+        }
     }
-    e.buf0 = buf0;
-    e.buf1 = buf1;
-    BatchState& s = b->h_state[j->slot];
-    s = BatchState{0u, 0u, 0u, 0u};
-    hipchk(hipMemcpyAsync(b->d_jobs + j->slot, &e, sizeof e, hipMemcpyHostToDevice, b->stream), "job table");
-    hipchk(hipMemcpyAsync(b->d_state + j->slot, &s, sizeof s, hipMemcpyHostToDevice, b->stream), "state table");
-    j->step = 0;
-    j->cur = 0;
-    j->window_end = 0;
 }
 
 // Coarse fields and state bytes of a finished job, then its completion event.
@@ -704,15 +832,9 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     for (auto& kv : cls) total += kv.second.size();
     const size_t bytes = total * sizeof(WorkItem) + entries.size() * 4 + 16;
     const int ib = b->items_buf ^= 1;
-    if (b->items_cap[ib] < bytes) {
-        // the buffer may still be read by queued rounds
-        hipchk(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
-        if (b->d_items[ib]) (void)hipFree(b->d_items[ib]);
-        b->d_items[ib] = nullptr;
-        const size_t cap = bytes + bytes / 2;
-        hipchk(hipMalloc((void**)&b->d_items[ib], cap), "hipMalloc(work items)");
-        b->items_cap[ib] = cap;
-    }
+    // at least 16 MiB, so a growing mix rarely drains the stream to grow it
+    grow_buffer(b, (char**)&b->d_items[ib], &b->items_cap[ib], std::max<size_t>(bytes, (size_t)16 << 20),
+                "hipMalloc(work items)");
     std::vector<WorkItem> all;
     all.reserve(total);
     b->classes.clear();
@@ -776,9 +898,10 @@ void draw_windows(lgca_batch* b, const std::vector<int>& ks)
 
 void start_tuning(lgca_batch* b, int user_k, uint32_t user_waves, uint64_t wr)
 {
+    b->evaluating = false;  // a restart abandons the previous measurement
     b->cands.clear();
     const int ks[2] = {4, 8};
-    const uint32_t ws[4] = {2048, 4096, 8192, 16384};
+    const uint32_t ws[5] = {2048, 4096, 8192, 16384, 32768};
     for (int k : ks) {
         if (user_k && k != user_k) continue;
         for (uint32_t w : ws) {
@@ -847,10 +970,14 @@ void run_round(lgca_batch* b)
     }
     if (b->evaluating) evaluate_tuning(b, false);
     const uint64_t wr_now = wave_rows(b, 4);
+    // A mix that has doubled since the last tuning (jobs still arriving) is
+    // tuned again at once; a shrinking one after 64 rounds.
+    const bool grown = b->tuned_wave_rows != 0 && wr_now > 2 * b->tuned_wave_rows;
     if (!b->tuning && !b->evaluating) {
-        const bool shifted = b->rounds_since_tune >= 64 &&
-                             (wr_now * 2 < b->tuned_wave_rows || wr_now > 2 * b->tuned_wave_rows);
-        if (retune || b->tuned_wave_rows == 0 || shifted) start_tuning(b, user_k, user_waves, wr_now);
+        const bool shrunk = b->rounds_since_tune >= 64 && wr_now * 2 < b->tuned_wave_rows;
+        if (retune || b->tuned_wave_rows == 0 || shrunk || grown) start_tuning(b, user_k, user_waves, wr_now);
+    } else if (grown || retune) {
+        start_tuning(b, user_k, user_waves, wr_now);
     }
     ++b->rounds_since_tune;
 
@@ -968,8 +1095,8 @@ void worker_main(lgca_batch* b)
                     b->cv_work.wait(lk);
                 }
             }
+            admit_all(b, admitted);
             for (Job* j : admitted) {
-                admit(b, j);
                 if (j->d.steps == 0) {
                     finish(b, j);
                     b->active_cells -= (uint64_t)j->d.dim_x * j->d.dim_y;
@@ -1078,7 +1205,7 @@ extern "C" lgca_status lgca_batch_destroy(lgca_batch* b)
     for (hipEvent_t e : b->staging_ev)
         if (e) (void)hipEventDestroy(e);
     b->arena.release_all();
-    void* dptrs[] = {b->d_jobs, b->d_state, b->d_items[0], b->d_items[1], b->d_draw, b->d_err};
+    void* dptrs[] = {b->d_jobs, b->d_state, b->d_items[0], b->d_items[1], b->d_draw, b->d_err, b->d_setup};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
     void* hptrs[] = {b->h_jobs, b->h_state, b->staging};

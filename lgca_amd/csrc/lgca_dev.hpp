@@ -170,6 +170,24 @@ __host__ __device__ __forceinline__ void apply_force(T s[7], T f, int dir)
     }
 }
 
+// Initial fill of plane d of word w of row y (hash stream 2): bit b is set
+// when philox({x >> 2, y, d, 2})[x & 3] < thr for x = 32w + b < dim_x.
+__device__ __forceinline__ uint32_t fill_word(uint32_t w, uint32_t y, uint32_t d, uint32_t dim_x,
+                                              uint32_t key0, uint32_t key1, uint64_t thr)
+{
+    uint32_t word = 0;
+    for (uint32_t gi = 0; gi < 8; ++gi) {
+        const uint32_t x = 32 * w + 4 * gi;
+        if (x >= dim_x) break;
+        const U4 r = philox4x32_10(x >> 2, y, d, 2u, key0, key1);
+        word |= (uint32_t)((uint64_t)r.x < thr) << (4 * gi + 0);
+        word |= (uint32_t)((uint64_t)r.y < thr) << (4 * gi + 1);
+        word |= (uint32_t)((uint64_t)r.z < thr) << (4 * gi + 2);
+        word |= (uint32_t)((uint64_t)r.w < thr) << (4 * gi + 3);
+    }
+    return word;
+}
+
 // Per-cell force decisions of word w of row y at `step`: 32 independent
 // Bernoulli(thr / 2^32) bits by first-fire sampling.  u0 is the word's first
 // uniform, philox({w, y, step >> 2, 1})[step & 3] (one call serves four

@@ -389,16 +389,7 @@ __global__ __launch_bounds__(256) void init_fill_kernel(uint32_t* __restrict__ p
         fluid &= sel;
     }
     for (uint32_t d = 0; d < g.Q; ++d) {
-        uint32_t word = 0;
-        for (uint32_t gi = 0; gi < 8; ++gi) {
-            const uint32_t x = 32 * w + 4 * gi;
-            if (x >= g.dim_x) break;
-            const U4 r = philox4x32_10(x >> 2, y, d, 2u, key0, key1);
-            word |= (uint32_t)((uint64_t)r.x < thr) << (4 * gi + 0);
-            word |= (uint32_t)((uint64_t)r.y < thr) << (4 * gi + 1);
-            word |= (uint32_t)((uint64_t)r.z < thr) << (4 * gi + 2);
-            word |= (uint32_t)((uint64_t)r.w < thr) << (4 * gi + 3);
-        }
+        const uint32_t word = fill_word(w, y, d, g.dim_x, key0, key1, thr);
         uint32_t* p = planes + ((size_t)y * g.Q + d) * g.P + w;
         *p = (word & fluid) | (*p & ~fluid);
     }
