@@ -164,6 +164,24 @@ LGCA_API lgca_status lgca_count_particles(lgca_lattice* h, uint64_t* count);
  * get_mean_velocity (src/omp_lattice.cpp:508-557).  Summed exactly. */
 LGCA_API lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2]);
 
+/* The pipe and karman viewers' forcing controller (apps/pipe/pipe_viewer.cpp:107-125,
+ * apps/karman/karman_viewer.cpp:108-124) on the GPU.  lgca_controller_set takes the
+ * force direction, the target velocity u (Lattice::u(), src/lattice.cpp:64-67) and the
+ * initial and equilibrium forcing amounts (get_initial_forcing /
+ * get_equilibrium_forcing, src/lattice.cpp:445-461).  lgca_controller_tick is one
+ * viewer tick: the mean x velocity of the output snapshot, as lgca_mean_velocity
+ * computes it; if it is below u, the forcing becomes the equilibrium amount once the
+ * velocity exceeds 0.9 u and is applied as lgca_apply_body_force_count does (same
+ * call counter); then `steps` steps and a new snapshot.  Nothing is read back, so
+ * ticks queue on the stream while the host post-processes earlier snapshots. */
+LGCA_API lgca_status lgca_controller_set(lgca_lattice* h, int dir, float u, int64_t forcing,
+                                         int64_t equilibrium_forcing);
+LGCA_API lgca_status lgca_controller_tick(lgca_lattice* h, uint64_t steps);
+/* Waits for the stream, then reports the controller's current forcing, the amount the
+ * last tick applied (-1: none) and the mean velocity that tick saw. */
+LGCA_API lgca_status lgca_controller_get(lgca_lattice* h, int64_t* forcing, int64_t* applied,
+                                         float velocity[2]);
+
 /* Checkpoint of the live state: model, sizes, seed, step, per-cell force
  * settings, the count-mode force call counter, cell types and state bytes, in
  * one file with a checksum.  A run resumed from it continues bit for bit as

@@ -93,6 +93,9 @@ SIGNATURES = {
                                  C.POINTER(C.c_void_p)],
     "lgca_count_particles": [_h, C.POINTER(C.c_uint64)],
     "lgca_mean_velocity": [_h, _f32p],
+    "lgca_controller_set": [_h, C.c_int, C.c_float, C.c_int64, C.c_int64],
+    "lgca_controller_tick": [_h, C.c_uint64],
+    "lgca_controller_get": [_h, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _f32p],
     "lgca_save_checkpoint": [_h, C.c_char_p],
     "lgca_load_checkpoint": [C.POINTER(_h), C.c_char_p, C.c_int],
     "lgca_batch_create": [C.POINTER(_h), C.c_int, C.c_uint64],

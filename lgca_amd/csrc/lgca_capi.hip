@@ -17,6 +17,18 @@
 
 using namespace lgca;
 
+namespace lgca {
+// Device-resident state of the viewers' forcing controller.
+struct Controller {
+    long long acc[3];   // mean-velocity sums of the snapshot: x, y (units of 1/840), FLUID cells
+    long long forcing;  // the controller's current forcing amount
+    long long amount;   // this tick: amount to apply, or -1
+    long long reverted; // particles reverted by the last applied force
+    uint32_t calls;     // count-mode force call counter (hash stream 3)
+    float v[2];         // mean velocity the last decision saw
+};
+}  // namespace lgca
+
 struct lgca_lattice {
     int model = 0;
     int kind = 0;
@@ -50,6 +62,12 @@ struct lgca_lattice {
     uint32_t tile_th = 0;           // rows per tile, 0 = automatic
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
+    // forcing controller of the viewers (lgca_controller_*)
+    Controller* ctrl = nullptr;
+    int ctrl_dir = LGCA_FORCE_NONE;
+    float ctrl_u = 0.0f;
+    int64_t ctrl_eq = 0;
+    bool calls_on_device = false;   // the count-mode call counter lives in ctrl
     std::string err;
     std::mutex mu;
 };
@@ -129,6 +147,19 @@ lgca_status clear_scratch(lgca_lattice* h)
     return LGCA_OK;
 }
 
+// Brings the count-mode force call counter back to the host after device
+// controller ticks advanced it.
+lgca_status sync_force_calls(lgca_lattice* h)
+{
+    if (!h->calls_on_device) return LGCA_OK;
+    uint32_t calls = 0;
+    HIPCHK(h, hipMemcpyAsync(&calls, &h->ctrl->calls, 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->force_calls = (h->force_calls & ~(uint64_t)0xFFFFFFFFu) | calls;
+    h->calls_on_device = false;
+    return LGCA_OK;
+}
+
 }  // namespace
 
 // -------------------------------------------------------------- count force
@@ -142,10 +173,9 @@ lgca_status clear_scratch(lgca_lattice* h)
 namespace lgca {
 
 template <int KIND>
-__global__ __launch_bounds__(64) void body_force_count_kernel(uint32_t* planes, const uint32_t* mask,
-                                                              Geometry g, uint32_t key0, uint32_t key1,
-                                                              uint32_t call, int dir, long long forcing,
-                                                              uint32_t* visited, long long* result)
+__device__ __forceinline__ void count_force_body(uint32_t* planes, const uint32_t* mask, const Geometry& g,
+                                                 uint32_t key0, uint32_t key1, uint32_t call, int dir,
+                                                 long long forcing, uint32_t* visited, long long* result)
 {
     constexpr int Q = num_dir_of_kind(KIND);
     const uint32_t lane = threadIdx.x;
@@ -211,6 +241,48 @@ __global__ __launch_bounds__(64) void body_force_count_kernel(uint32_t* planes, 
         if (hits) break;
     }
     if (lane == 0) *result = reverted;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void body_force_count_kernel(uint32_t* planes, const uint32_t* mask,
+                                                              Geometry g, uint32_t key0, uint32_t key1,
+                                                              uint32_t call, int dir, long long forcing,
+                                                              uint32_t* visited, long long* result)
+{
+    count_force_body<KIND>(planes, mask, g, key0, key1, call, dir, forcing, visited, result);
+}
+
+// The same, with the amount and call number the controller left on the GPU;
+// nothing happens (and no call is counted) when the amount is negative.
+template <int KIND>
+__global__ __launch_bounds__(64) void controlled_force_kernel(uint32_t* planes, const uint32_t* mask,
+                                                              Geometry g, uint32_t key0, uint32_t key1,
+                                                              int dir, Controller* c, uint32_t* visited)
+{
+    const long long amount = c->amount;
+    if (amount < 0) return;
+    const uint32_t call = c->calls;
+    count_force_body<KIND>(planes, mask, g, key0, key1, call, dir, amount, visited, &c->reverted);
+    if (threadIdx.x == 0) c->calls = call + 1;
+}
+
+// The viewers' decision (apps/pipe/pipe_viewer.cpp:107-125): the mean x
+// velocity of the snapshot, in the FP32 value lgca_mean_velocity returns; if
+// it is below u, the forcing becomes `eq` once it exceeds 0.9 u (compared in
+// double, as the viewer's float * double promotes), and the forcing applies.
+__global__ void controller_decide_kernel(Controller* c, float u, long long eq, double unit_y)
+{
+    const double nf = (double)c->acc[2];
+    const float v0 = nf > 0 ? (float)((double)c->acc[0] / (840.0 * nf)) : 0.0f;
+    const float v1 = nf > 0 ? (float)((double)c->acc[1] * unit_y / (840.0 * nf)) : 0.0f;
+    c->v[0] = v0;
+    c->v[1] = v1;
+    long long amount = -1;
+    if (v0 < u) {
+        if ((double)v0 > 0.9 * (double)u) c->forcing = eq;
+        amount = c->forcing;
+    }
+    c->amount = amount;
 }
 
 }  // namespace lgca
@@ -284,7 +356,7 @@ extern "C" lgca_status lgca_destroy(lgca_lattice* h)
         if (h->stream && h->stream != h->own_stream) (void)hipStreamSynchronize(h->stream);
         void* ptrs[] = {h->state[0], h->state[1], h->snap, h->mask, h->row_range, h->dec,
                         h->cell_density, h->cell_momentum, h->mean_density, h->mean_momentum,
-                        h->velocity, h->bytes, h->visited, h->scratch};
+                        h->velocity, h->bytes, h->visited, h->scratch, h->ctrl};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
@@ -604,11 +676,19 @@ extern "C" lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint
     return LGCA_OK;
 }
 
+static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps);
+
 extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
 {
     if (!h) return LGCA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
+    return step_locked(h, nsteps);
+}
+
+// The body of lgca_step; the caller holds h->mu and the device.
+static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps)
+{
     const bool forcing = h->force_dir != LGCA_FORCE_NONE;
     const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
     const size_t per_step = (size_t)(fw1 - fw0) * (h->fy1 - h->fy0);
@@ -681,6 +761,8 @@ extern "C" lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int
     if (!h || (dir != LGCA_FORCE_X && dir != LGCA_FORCE_Y)) return LGCA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
+    lgca_status st0 = sync_force_calls(h);
+    if (st0) return st0;
     const uint64_t n = (uint64_t)h->dim_x * h->dim_y;
     const uint64_t vw = (n + 31) / 32;
     if (!h->visited) HIPCHK(h, dalloc(&h->visited, vw * 4));
@@ -845,6 +927,8 @@ extern "C" lgca_status lgca_save_checkpoint(lgca_lattice* h, const char* path)
     if (!h || !path) return LGCA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
+    lgca_status st0 = sync_force_calls(h);
+    if (st0) return st0;
     const size_t n = (size_t)h->dim_x * h->dim_y;
     std::string buf(2 * n, '\0');
     uint8_t* types = (uint8_t*)&buf[0];
@@ -932,5 +1016,86 @@ extern "C" lgca_status lgca_load_checkpoint(lgca_lattice** out, const char* path
         return st;
     }
     *out = h;
+    return LGCA_OK;
+}
+
+// ------------------------------------------------------- forcing controller
+
+extern "C" lgca_status lgca_controller_set(lgca_lattice* h, int dir, float u, int64_t forcing,
+                                           int64_t equilibrium_forcing)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    if (dir != LGCA_FORCE_X && dir != LGCA_FORCE_Y)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_controller_set: dir must be X or Y");
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    lgca_status st = sync_force_calls(h);
+    if (st) return st;
+    if (!h->ctrl) HIPCHK(h, dalloc(&h->ctrl, sizeof(Controller)));
+    HIPCHK(h, launch_fill_u32((uint32_t*)h->ctrl, 0u, sizeof(Controller) / 4, h->stream));
+    // forcing (64-bit) as two words, written by fill kernels: no host buffer
+    // has to outlive the call
+    uint32_t* f = (uint32_t*)&h->ctrl->forcing;
+    HIPCHK(h, launch_fill_u32(f, (uint32_t)(uint64_t)forcing, 1, h->stream));
+    HIPCHK(h, launch_fill_u32(f + 1, (uint32_t)((uint64_t)forcing >> 32), 1, h->stream));
+    h->ctrl_dir = dir;
+    h->ctrl_u = u;
+    h->ctrl_eq = equilibrium_forcing;
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_controller_tick(lgca_lattice* h, uint64_t steps)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ctrl) return fail(h, LGCA_ERR_INVALID_STATE, "lgca_controller_tick: no lgca_controller_set yet");
+    DeviceGuard dg(h->device);
+    if (!h->calls_on_device) {
+        HIPCHK(h, launch_fill_u32(&h->ctrl->calls, (uint32_t)h->force_calls, 1, h->stream));
+        h->calls_on_device = true;
+    }
+    // decision on the snapshot's mean velocity
+    HIPCHK(h, launch_fill_u32((uint32_t*)h->ctrl->acc, 0u, 6, h->stream));
+    HIPCHK(h, launch_mean_velocity(h->kind, h->snap, h->mask, h->g, h->ctrl->acc, h->stream));
+    const double unit_y = h->kind == kHPP ? 1.0 : (double)0.866025388240814208984375f;
+    hipLaunchKernelGGL(controller_decide_kernel, dim3(1), dim3(1), 0, h->stream, h->ctrl, h->ctrl_u,
+                       (long long)h->ctrl_eq, unit_y);
+    HIPCHK(h, hipGetLastError());
+    // count-mode force with the decided amount
+    const uint64_t n = (uint64_t)h->dim_x * h->dim_y;
+    const uint64_t vw = (n + 31) / 32;
+    if (!h->visited) HIPCHK(h, dalloc(&h->visited, vw * 4));
+    HIPCHK(h, launch_fill_u32(h->visited, 0u, vw, h->stream));
+    const uint32_t k0 = (uint32_t)h->seed, k1 = (uint32_t)(h->seed >> 32);
+    switch (h->kind) {
+    case kHPP: hipLaunchKernelGGL(controlled_force_kernel<kHPP>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, h->ctrl_dir, h->ctrl, h->visited); break;
+    case kFHP1: hipLaunchKernelGGL(controlled_force_kernel<kFHP1>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, h->ctrl_dir, h->ctrl, h->visited); break;
+    default: hipLaunchKernelGGL(controlled_force_kernel<kFHP2>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, h->ctrl_dir, h->ctrl, h->visited); break;
+    }
+    HIPCHK(h, hipGetLastError());
+    // the steps, then the snapshot the next tick decides on
+    lgca_status st = step_locked(h, steps);
+    if (st) return st;
+    HIPCHK(h, hipMemcpyAsync(h->snap, h->state[h->cur], h->state_words * 4, hipMemcpyDeviceToDevice,
+                             h->stream));
+    return LGCA_OK;
+}
+
+extern "C" lgca_status lgca_controller_get(lgca_lattice* h, int64_t* forcing, int64_t* applied,
+                                           float velocity[2])
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->ctrl) return fail(h, LGCA_ERR_INVALID_STATE, "lgca_controller_get: no lgca_controller_set yet");
+    DeviceGuard dg(h->device);
+    Controller c;
+    HIPCHK(h, hipMemcpyAsync(&c, h->ctrl, sizeof c, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (forcing) *forcing = c.forcing;
+    if (applied) *applied = c.amount;
+    if (velocity) {
+        velocity[0] = c.v[0];
+        velocity[1] = c.v[1];
+    }
     return LGCA_OK;
 }

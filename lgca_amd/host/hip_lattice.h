@@ -266,6 +266,19 @@ public:
         check(lgca_set_forcing(m_h, dir == 'x' ? LGCA_FORCE_X : (dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_NONE),
                                p, x0, x1, y0, y1));
     }
+    // The viewer's forcing controller on the GPU: a tick decides on the
+    // snapshot's mean velocity, applies the count-mode force, runs `steps`
+    // steps and snapshots, without the host waiting (lgca_controller_*).
+    void set_forcing_controller(int64_t forcing, int64_t equilibrium)
+    {
+        check(lgca_controller_set(m_h, m_bf_dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_X, m_u, forcing,
+                                  equilibrium));
+    }
+    void controlled_tick(uint64_t steps)
+    {
+        sync_types();
+        check(lgca_controller_tick(m_h, steps));
+    }
     // Checkpoint of state, types, seed, step and forcing; resume with
     // lgca_load_checkpoint (include/lgca.h).
     void save_checkpoint(const std::string& path)

@@ -123,6 +123,30 @@ class Lattice:
         self.m_cell_type_cpu[:] = self.get_cell_types()
         return self
 
+    def set_forcing_controller(self, forcing: int | None = None, equilibrium: int | None = None,
+                               u: float | None = None) -> None:
+        """The viewers' forcing controller on the GPU (apps/pipe/pipe_viewer.cpp:107-125):
+        direction m_bf_dir, target velocity u (default u()), initial and equilibrium
+        amounts (defaults: get_initial_forcing() / get_equilibrium_forcing())."""
+        d = {"x": capi.FORCE_X, "y": capi.FORCE_Y}[self.m_bf_dir or "x"]
+        f0 = self.get_initial_forcing() if forcing is None else forcing
+        eq = self.get_equilibrium_forcing() if equilibrium is None else equilibrium
+        uu = float(self.u()) if u is None else float(f32(u))
+        self._check(lib().lgca_controller_set(self._h, d, uu, int(f0), int(eq)))
+
+    def controlled_tick(self, steps: int) -> None:
+        """One viewer tick on the GPU: decide on the snapshot's mean velocity,
+        apply the count-mode force, run `steps` steps, take a new snapshot."""
+        self.copy_data_to_device()
+        self._check(lib().lgca_controller_tick(self._h, int(steps)))
+
+    def controller_state(self):
+        """(forcing, amount applied by the last tick or -1, (vx, vy) it saw)."""
+        f, a = C.c_int64(), C.c_int64()
+        v = (C.c_float * 2)()
+        self._check(lib().lgca_controller_get(self._h, C.byref(f), C.byref(a), v))
+        return int(f.value), int(a.value), (float(v[0]), float(v[1]))
+
     def save_checkpoint(self, path: str) -> None:
         self.copy_data_to_device()
         self._check(lib().lgca_save_checkpoint(self._h, os.fsencode(path)))
