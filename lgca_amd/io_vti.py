@@ -1,13 +1,14 @@
 """VTK XML image files (.vti) of a lattice's output arrays, without VTK.
 
-The Python twin of lgca_amd/host/io_vti.h (the two write identical bytes).
+The Python twin of lgca_amd/host/io_vti.h (uncompressed, the two write
+identical bytes).
 `write_lattice_vti` produces the pair of files the reference's IoVti<Model>
 writes (src/lgca_io_vti.cpp:113-144): cell_res_<step>.vti with
 (dim_x + 1) x (dim_y + 1) points and CellData "Cell density" / "Cell
 momentum", and mean_res_<step>.vti with coarse_dim_x x coarse_dim_y points
 and PointData "Mean density" / "Mean momentum" (src/lgca_io_vti.cpp:50-87).
-Arrays are raw appended binary with UInt64 byte-count headers instead of the
-reference's LZ4 blocks.  `read_vti` parses such files back.
+Arrays are raw appended binary with UInt64 byte-count headers (the C++ writer
+also has the reference's LZ4 compression).  `read_vti` parses both back.
 """
 from __future__ import annotations
 
@@ -62,13 +63,40 @@ def write_lattice_vti(lattice, step: int, directory: str, scalars: str = "Mean d
     return cell, mean
 
 
+_liblz4 = None
+
+
+def lz4_decompress_block(data: bytes, size: int) -> bytes:
+    """One LZ4 block, decoded by the system's liblz4 (an independent decoder
+    of what lgca_amd/host/io_vti.h encodes)."""
+    global _liblz4
+    if _liblz4 is None:
+        import ctypes
+        import ctypes.util
+        name = ctypes.util.find_library("lz4") or "liblz4.so.1"
+        _liblz4 = ctypes.CDLL(name)
+        _liblz4.LZ4_decompress_safe.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                                 ctypes.c_int]
+        _liblz4.LZ4_decompress_safe.restype = ctypes.c_int
+    import ctypes
+    out = ctypes.create_string_buffer(max(size, 1))
+    got = _liblz4.LZ4_decompress_safe(data, out, len(data), size)
+    if got != size:
+        raise ValueError(f"LZ4 block decoded to {got} bytes, expected {size}")
+    return out.raw[:size]
+
+
 def read_vti(path: str) -> dict:
     """{'extent': (x0, x1, y0, y1, z0, z1), 'section': 'CellData'|'PointData',
-    'scalars': name or None, 'arrays': {name: float32 array (tuples, comps)}}."""
+    'scalars': name or None, 'arrays': {name: float32 array (tuples, comps)},
+    'compressor': name or None}.  Reads raw or LZ4-compressed appended data."""
     raw = open(path, "rb").read()
     cut = raw.index(b'<AppendedData encoding="raw">')
     start = raw.index(b"_", cut) + 1
     xml = raw[:cut].decode()
+    comp = re.search(r'compressor="([^"]+)"', xml)
+    if comp and comp.group(1) != "vtkLZ4DataCompressor":
+        raise ValueError(f"unsupported compressor {comp.group(1)}")
     ext = tuple(int(v) for v in re.search(r'WholeExtent="([^"]+)"', xml).group(1).split())
     sec = re.search(r"<(CellData|PointData)([^>]*)>", xml)
     scal = re.search(r'Scalars="([^"]+)"', sec.group(2))
@@ -77,8 +105,19 @@ def read_vti(path: str) -> dict:
                          r'format="appended" offset="(\d+)"/>', xml):
         name, comps, off = m.group(1), int(m.group(2)), int(m.group(3))
         pos = start + off
-        n = int(np.frombuffer(raw, dtype=np.uint64, count=1, offset=pos)[0])
-        a = np.frombuffer(raw, dtype=np.float32, count=n // 4, offset=pos + 8)
+        if comp:
+            nb, bs, last = (int(v) for v in np.frombuffer(raw, dtype=np.uint64, count=3, offset=pos))
+            sizes = np.frombuffer(raw, dtype=np.uint64, count=nb, offset=pos + 24)
+            p = pos + 24 + 8 * nb
+            parts = []
+            for i in range(nb):
+                usize = last if (i == nb - 1 and last) else bs
+                parts.append(lz4_decompress_block(raw[p:p + int(sizes[i])], usize))
+                p += int(sizes[i])
+            a = np.frombuffer(b"".join(parts), dtype=np.float32)
+        else:
+            n = int(np.frombuffer(raw, dtype=np.uint64, count=1, offset=pos)[0])
+            a = np.frombuffer(raw, dtype=np.float32, count=n // 4, offset=pos + 8)
         arrays[name] = a.reshape(-1, comps)
     return {"extent": ext, "section": sec.group(1), "scalars": scal.group(1) if scal else None,
-            "arrays": arrays}
+            "arrays": arrays, "compressor": comp.group(1) if comp else None}

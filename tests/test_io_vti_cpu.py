@@ -48,7 +48,7 @@ def test_python_writer_layout(tmp_path):
     assert int(np.frombuffer(raw, dtype=np.uint64, count=1, offset=cut)[0]) == 6 * 4
 
 
-def test_cpp_and_python_writers_agree(tmp_path):
+def _writer(tmp_path):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
@@ -58,9 +58,23 @@ def test_cpp_and_python_writers_agree(tmp_path):
                         os.path.join(ROOT, "tests", "native", "vti_writer.cpp"), "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    return exe
+
+
+def _have_liblz4():
+    try:
+        from lgca_amd.io_vti import lz4_decompress_block
+        lz4_decompress_block(b"\x10a", 1)
+        return True
+    except OSError:
+        return False
+
+
+def test_cpp_and_python_writers_agree(tmp_path):
+    exe = _writer(tmp_path)
     out = tmp_path / "cpp"
     out.mkdir()
-    subprocess.run([str(exe), str(out) + "/"], check=True)
+    subprocess.run([str(exe), str(out), "raw"], check=True)
     from lgca_amd.io_vti import write_lattice_vti
     py = tmp_path / "py"
     py.mkdir()
@@ -74,3 +88,43 @@ def test_empty_scalars_and_single_component(tmp_path):
     write_vti(str(p), 2, 2, False, [("a", np.array([1.0, 2.0, 3.0, 4.0], np.float32), 1)], "")
     v = read_vti(str(p))
     assert v["scalars"] is None and v["arrays"]["a"].shape == (4, 1)
+
+
+def test_lz4_blocks_decode_with_the_reference_decoder(tmp_path):
+    """Every block the encoder writes decodes, with the system's liblz4, to the
+    bytes it encoded: sizes 0..40, random, zero, periodic and float data."""
+    if not _have_liblz4():
+        pytest.skip("liblz4 not available")
+    from lgca_amd.io_vti import lz4_decompress_block
+    exe = _writer(tmp_path)
+    subprocess.run([str(exe), str(tmp_path), "blocks"], check=True)
+    raw = (tmp_path / "blocks.bin").read_bytes()
+    pos, count, saved = 0, 0, 0
+    while pos < len(raw):
+        n, c = (int(v) for v in np.frombuffer(raw, dtype=np.uint64, count=2, offset=pos))
+        pos += 16
+        src = raw[pos:pos + n]
+        pos += n
+        blk = raw[pos:pos + c]
+        pos += c
+        assert lz4_decompress_block(blk, n) == src, (count, n, c)
+        saved += n - c
+        count += 1
+    assert count == 46 and saved > 32768 * 2
+
+
+def test_lz4_files_match_the_raw_arrays(tmp_path):
+    if not _have_liblz4():
+        pytest.skip("liblz4 not available")
+    exe = _writer(tmp_path)
+    for mode in ("raw", "lz4"):
+        (tmp_path / mode).mkdir()
+        subprocess.run([str(exe), str(tmp_path / mode), mode], check=True)
+    for name in ("cell_res_7.vti", "mean_res_7.vti"):
+        a = read_vti(str(tmp_path / "raw" / name))
+        b = read_vti(str(tmp_path / "lz4" / name))
+        assert a["compressor"] is None and b["compressor"] == "vtkLZ4DataCompressor"
+        assert a["extent"] == b["extent"] and a["scalars"] == b["scalars"]
+        assert list(a["arrays"]) == list(b["arrays"])
+        for k in a["arrays"]:
+            assert np.array_equal(a["arrays"][k], b["arrays"][k]), k
