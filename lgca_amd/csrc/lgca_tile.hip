@@ -43,12 +43,30 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane i gets lane 
 
 struct LaneCtx {
     uint32_t w;          // word index of this lane (torus, 0..W-1)
+    uint32_t wb;         // 4 * w, the lane's byte offset within a plane row
     uint32_t first_shl;  // 32 - nb on the lane holding word 0, else 0
     bool last;           // lane holds word W-1
     uint32_t nb;
     uint32_t vmask;      // valid bits of this lane's word
     uint32_t lsrc, rsrc; // segment mode: byte address of the lanes holding words w-1, w+1
 };
+
+// Memory goes through buffer resources: a wave-uniform base in scalar
+// registers, a 32-bit per-lane offset and a scalar offset, so no load or
+// store needs vector address arithmetic, and the hardware drops accesses past
+// `bytes`.  (Word 3 selects raw 32-bit access, as for every gfx9 buffer.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+}
 
 // Neighbouring words: the adjacent lane (DPP) when a wave holds consecutive
 // words of one row, or the lane holding word w -+ 1 of the same row in
@@ -131,6 +149,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         c.lsrc = c.rsrc = 0;
     }
     c.w = (uint32_t)((wabs + (int)g.W) % (int)g.W);
+    c.wb = 4u * c.w;
     c.nb = g.nb;
     c.first_shl = (c.w == 0) ? 32u - g.nb : 0u;
     c.last = c.w + 1 == g.W;
@@ -142,6 +161,24 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     const uint32_t* src = a.src;
     uint32_t* dst = a.dst;
     const uint32_t n_iter = (SEG ? a.th : rows) + 2 * K;
+
+    // Plane d (or mask plane b) of row r at this lane's word.  Strip mode: the
+    // row is wave-uniform, so each access builds a descriptor over that row.
+    // Segment mode: rows differ per lane, so one descriptor spans the buffer
+    // (W <= 64 keeps it under 4 GiB) and the row goes into the lane offset.
+    const uint32_t plane_b = g.P * 4u, row_b = (uint32_t)Q * plane_b;
+    auto load_plane = [&](uint32_t r, int d) -> uint32_t {
+        if (SEG) return bload(make_rsrc(src, g.dim_y * row_b), r * row_b + c.wb, (uint32_t)d * plane_b);
+        return bload(make_rsrc(src + (size_t)r * Q * g.P, row_b), c.wb, (uint32_t)d * plane_b);
+    };
+    auto store_plane = [&](uint32_t r, int d, uint32_t v) {
+        if (SEG) bstore(v, make_rsrc(dst, g.dim_y * row_b), r * row_b + c.wb, (uint32_t)d * plane_b);
+        else bstore(v, make_rsrc(dst + (size_t)r * Q * g.P, row_b), c.wb, (uint32_t)d * plane_b);
+    };
+    auto load_mask = [&](uint32_t r, int b) -> uint32_t {
+        if (SEG) return bload(make_rsrc(a.mask, g.dim_y * 2u * plane_b), r * 2u * plane_b + c.wb, (uint32_t)b * plane_b);
+        return bload(make_rsrc(a.mask + (size_t)r * 2 * g.P, 2u * plane_b), c.wb, (uint32_t)b * plane_b);
+    };
     const int dy = (int)g.dim_y;
     auto wrap = [&](int r) -> uint32_t {
         if (r < 0) {
@@ -220,12 +257,11 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
     {
         if (tile_solids && (SEG || row_has_solids(row_i))) {
-            nns = a.mask[((size_t)row_i * 2 + 0) * g.P + c.w];
-            nsl = a.mask[((size_t)row_i * 2 + 1) * g.P + c.w];
+            nns = load_mask(row_i, 0);
+            nsl = load_mask(row_i, 1);
         }
-        const uint32_t* p = src + (size_t)row_i * Q * g.P;
 #pragma unroll
-        for (int d = 0; d < Q; ++d) nxt[d] = p[(size_t)d * g.P + c.w];
+        for (int d = 0; d < Q; ++d) nxt[d] = load_plane(row_i, d);
     }
 
     // One step of the row walk.  STEADY iterations (i >= 2K) have every level
@@ -249,8 +285,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
             const uint32_t rn = row_next;
             nns = nsl = 0;
             if (tile_solids && (SEG || row_has_solids(rn))) {
-                nns = a.mask[((size_t)rn * 2 + 0) * g.P + c.w];
-                nsl = a.mask[((size_t)rn * 2 + 1) * g.P + c.w];
+                nns = load_mask(rn, 0);
+                nsl = load_mask(rn, 1);
             }
             if (tile_force) {
 #pragma unroll
@@ -264,9 +300,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                                           (size_t)(jn - a.fy0) * a.fwn + (c.w - a.fw0)];
                 }
             }
-            const uint32_t* p = src + (size_t)rn * Q * g.P;
 #pragma unroll
-            for (int d = 0; d < Q; ++d) nxt[d] = p[(size_t)d * g.P + c.w];
+            for (int d = 0; d < Q; ++d) nxt[d] = load_plane(rn, d);
         }
 
 #pragma unroll
@@ -357,9 +392,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                 for (int d = 0; d < Q; ++d) cur[d] = out[d];
                 // segment mode: rows past a segment's own tile belong to the next
                 if (s == K && writer && (!SEG || i - (uint32_t)(2 * K) < rows)) {
-                    uint32_t* q = dst + (size_t)j * Q * g.P;
 #pragma unroll
-                    for (int d = 0; d < Q; ++d) q[(size_t)d * g.P + c.w] = cur[d];
+                    for (int d = 0; d < Q; ++d) store_plane(j, d, cur[d]);
                 }
             }
         }
