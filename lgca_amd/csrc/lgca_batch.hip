@@ -352,7 +352,7 @@ struct Arena {
 
 struct ClassRange {
     int kind, k;
-    bool ragged, seg;
+    bool ragged;
     uint32_t offset, count;
 };
 
@@ -801,7 +801,7 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     for (size_t i = 0; i < n; ++i) b->sig.push_back(((uint64_t)b->active[i]->slot << 8) | (uint64_t)ks[i]);
     if (b->sig == b->last_sig) return;
     b->last_sig = b->sig;
-    // classes in a fixed order: kind, k, ragged, segment mode
+    // classes in a fixed order: kind, k, ragged (segment and strip jobs share)
     std::map<uint64_t, std::vector<WorkItem>> cls;
     std::vector<uint32_t> entries;
     entries.reserve(n);
@@ -813,13 +813,12 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         const uint32_t halo = seg ? 0u : tile_halo(g, k);
         const uint32_t jth = std::min<uint32_t>(th, g.dim_y + (g.dim_y & 1u));
         const uint32_t tiles = (g.dim_y + jth - 1) / jth;
-        const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) |
-                             (g.nb != 32 ? 2u : 0u) | (seg ? 1u : 0u);
+        const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) | (g.nb != 32 ? 1u : 0u);
         std::vector<WorkItem>& v = cls[key];
         if (seg) {
             const uint32_t S = 64 / g.W;
             for (uint32_t t = 0; t < (tiles + S - 1) / S; ++t)
-                v.push_back(WorkItem{(uint32_t)j->slot, 0u, t, jth});
+                v.push_back(WorkItem{(uint32_t)j->slot, 0u, t, jth | (1u << 31)});
         } else {
             const uint32_t strips = tile_strips(g, k);
             for (uint32_t t = 0; t < tiles; ++t)
@@ -842,8 +841,7 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         ClassRange c;
         c.kind = (int)(kv.first >> 16);
         c.k = (int)((kv.first >> 8) & 0xFF);
-        c.ragged = (kv.first & 2u) != 0;
-        c.seg = (kv.first & 1u) != 0;
+        c.ragged = (kv.first & 1u) != 0;
         c.offset = (uint32_t)all.size();
         c.count = (uint32_t)kv.second.size();
         b->classes.push_back(c);
@@ -901,7 +899,7 @@ void start_tuning(lgca_batch* b, int user_k, uint32_t user_waves, uint64_t wr)
     b->evaluating = false;  // a restart abandons the previous measurement
     b->cands.clear();
     const int ks[2] = {4, 8};
-    const uint32_t ws[5] = {2048, 4096, 8192, 16384, 32768};
+    const uint32_t ws[4] = {2048, 4096, 8192, 16384};
     for (int k : ks) {
         if (user_k && k != user_k) continue;
         for (uint32_t w : ws) {
@@ -971,12 +969,13 @@ void run_round(lgca_batch* b)
     if (b->evaluating) evaluate_tuning(b, false);
     const uint64_t wr_now = wave_rows(b, 4);
     // A mix that has doubled since the last tuning (jobs still arriving) is
-    // tuned again at once; a shrinking one after 64 rounds.
-    const bool grown = b->tuned_wave_rows != 0 && wr_now > 2 * b->tuned_wave_rows;
+    // tuned again at once, a shrinking one after 64 rounds; a tuning under way
+    // restarts only when the mix has grown 4x since it began.
     if (!b->tuning && !b->evaluating) {
+        const bool grown = b->tuned_wave_rows != 0 && wr_now > 2 * b->tuned_wave_rows;
         const bool shrunk = b->rounds_since_tune >= 64 && wr_now * 2 < b->tuned_wave_rows;
         if (retune || b->tuned_wave_rows == 0 || shrunk || grown) start_tuning(b, user_k, user_waves, wr_now);
-    } else if (grown || retune) {
+    } else if (retune || wr_now > 4 * b->tuned_wave_rows) {
         start_tuning(b, user_k, user_waves, wr_now);
     }
     ++b->rounds_since_tune;
@@ -997,7 +996,7 @@ void run_round(lgca_batch* b)
         hipchk(hipEventRecord(b->tune_ev[2 * b->cand_i], b->stream), "hipEventRecord");
     uint64_t nl = 0;
     for (const ClassRange& c : b->classes) {
-        hipchk(launch_tile_batch(c.kind, c.k, c.ragged, c.seg, b->d_jobs, b->d_state,
+        hipchk(launch_tile_batch(c.kind, c.k, c.ragged, b->d_jobs, b->d_state,
                                  b->d_items[b->items_buf] + c.offset, c.count, b->stream),
                "tile_batch_kernel");
         ++nl;

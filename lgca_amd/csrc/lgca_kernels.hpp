@@ -84,12 +84,12 @@ struct BatchState {
 };
 struct WorkItem {
     uint32_t job, strip, tile;  // segment mode: strip 0, tile = group of 64 / W row tiles
-    uint32_t shape;        // rows per tile | halo << 16
+    uint32_t shape;        // rows per tile | halo << 16 | segment mode << 31
 };
 // One class launch of a round: every item (all of one model, K and raggedness)
 // advances its job K steps reading its BatchState.  launch_advance then moves
 // each listed job (entry = slot | k << 24) k steps on and swaps its buffers.
-hipError_t launch_tile_batch(int kind, int k, bool ragged, bool seg, const BatchJob* jobs,
+hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
                              const BatchState* state, const WorkItem* items, uint32_t n_items,
                              hipStream_t s);
 hipError_t launch_advance(BatchState* state, const uint32_t* entries, uint32_t n, hipStream_t s);

@@ -463,8 +463,9 @@ void tile_kernel(TileArgs a)
 // runs that job's lattice.  A job's step, current buffer and force-window
 // origin live in device memory (advance_kernel moves them after each round),
 // so queued rounds need no host-to-device traffic.  One launch covers the
-// items of one (model, K, ragged, segment mode) class.
-template <int KIND, int K, bool RAGGED, bool SEG>
+// items of one (model, K, ragged) class; the top bit of an item's shape picks
+// segment mode for its job, so strip and segment jobs share the launch.
+template <int KIND, int K, bool RAGGED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __restrict__ state,
                        const WorkItem* __restrict__ items, uint32_t n_items)
@@ -478,11 +479,12 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __re
     TileArgs a = jobs[it.job].a;
     a.t0 = bs.step;
     a.th = it.shape & 0xFFFFu;
-    a.halo = it.shape >> 16;
+    a.halo = (it.shape >> 16) & 0x7FFFu;
     a.src = bs.cur ? jobs[it.job].buf1 : jobs[it.job].buf0;
     a.dst = bs.cur ? jobs[it.job].buf0 : jobs[it.job].buf1;
     if (a.fdec) a.fdec += (size_t)(bs.step - bs.fdec_step0) * a.fdec_stride;
-    tile_body<KIND, K, RAGGED, SEG>(a, it.strip, it.tile);
+    if (it.shape >> 31) tile_body<KIND, K, RAGGED, true>(a, it.strip, it.tile);
+    else tile_body<KIND, K, RAGGED, false>(a, it.strip, it.tile);
 }
 
 __global__ __launch_bounds__(256) void advance_kernel(BatchState* state, const uint32_t* entries,
@@ -496,31 +498,24 @@ __global__ __launch_bounds__(256) void advance_kernel(BatchState* state, const u
     s.cur ^= 1u;
 }
 
-hipError_t launch_tile_batch(int kind, int k, bool ragged, bool seg, const BatchJob* jobs,
+hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
                              const BatchState* state, const WorkItem* items, uint32_t n_items,
                              hipStream_t s)
 {
     if (!n_items) return hipSuccess;
     const dim3 grid((n_items + 3) / 4);
-#define LGCA_BATCH_LAUNCH(KD, KK, RG, SG)                                                        \
-    hipLaunchKernelGGL((tile_batch_kernel<KD, KK, RG, SG>), grid, dim3(256), 0, s, jobs, state,  \
-                       items, n_items)
-#define LGCA_BATCH_CASE(KD, KK)                                                                  \
-    if (kind == KD && k == KK) {                                                               \
-        if (ragged) {                                                                          \
-            if (seg) LGCA_BATCH_LAUNCH(KD, KK, true, true);                                    \
-            else LGCA_BATCH_LAUNCH(KD, KK, true, false);                                       \
-        } else {                                                                               \
-            if (seg) LGCA_BATCH_LAUNCH(KD, KK, false, true);                                   \
-            else LGCA_BATCH_LAUNCH(KD, KK, false, false);                                      \
-        }                                                                                      \
-        return hipGetLastError();                                                              \
+#define LGCA_BATCH_CASE(KD, KK)                                                                   \
+    if (kind == KD && k == KK) {                                                                \
+        if (ragged) hipLaunchKernelGGL((tile_batch_kernel<KD, KK, true>), grid, dim3(256), 0, s,  \
+                                       jobs, state, items, n_items);                            \
+        else hipLaunchKernelGGL((tile_batch_kernel<KD, KK, false>), grid, dim3(256), 0, s, jobs,  \
+                                state, items, n_items);                                         \
+        return hipGetLastError();                                                               \
     }
     LGCA_BATCH_CASE(kHPP, 1) LGCA_BATCH_CASE(kHPP, 2) LGCA_BATCH_CASE(kHPP, 4) LGCA_BATCH_CASE(kHPP, 8)
     LGCA_BATCH_CASE(kFHP1, 1) LGCA_BATCH_CASE(kFHP1, 2) LGCA_BATCH_CASE(kFHP1, 4) LGCA_BATCH_CASE(kFHP1, 8)
     LGCA_BATCH_CASE(kFHP2, 1) LGCA_BATCH_CASE(kFHP2, 2) LGCA_BATCH_CASE(kFHP2, 4) LGCA_BATCH_CASE(kFHP2, 8)
 #undef LGCA_BATCH_CASE
-#undef LGCA_BATCH_LAUNCH
     return hipErrorInvalidValue;
 }
 
