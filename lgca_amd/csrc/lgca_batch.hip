@@ -149,16 +149,28 @@ __global__ __launch_bounds__(256) void batch_draw_kernel(const BatchJob* __restr
     const uint32_t step0 = state[slot].step;
     const uint32_t rows = j.a.fy1 - j.a.fy0, fwn = j.a.fwn;
     const uint32_t groups = ((step0 & 3u) + nsteps + 3u) / 4u;
-    const uint64_t n = (uint64_t)groups * rows * fwn;
+    const uint64_t plane = (uint64_t)rows * fwn;
+    const uint64_t n = groups * plane;
     uint32_t* dec = const_cast<uint32_t*>(j.a.fdec);
-    for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < n;
-         idx += (uint64_t)gridDim.x * 256) {
-        const uint32_t ww = (uint32_t)(idx % fwn);
-        const uint32_t yy = (uint32_t)((idx / fwn) % rows);
-        const uint32_t g = (uint32_t)(idx / ((uint64_t)fwn * rows));
-        const uint32_t w = j.a.fw0 + ww;
-        draw_group(dec + (size_t)yy * fwn + ww, (uint64_t)rows * fwn, nsteps, step0, g, w, j.a.fy0 + yy,
-                   j.a.key0, j.a.key1, Qt, force_columns(w, j.fx0, j.fx1, j.a.g.dim_x));
+    if (n < ((uint64_t)1 << 32)) {  // 32-bit divisions, the usual case
+        const uint32_t p32 = (uint32_t)plane;
+        for (uint32_t idx = blockIdx.x * 256 + threadIdx.x; idx < (uint32_t)n; idx += gridDim.x * 256) {
+            const uint32_t g = idx / p32, rem = idx - g * p32;
+            const uint32_t yy = rem / fwn, ww = rem - yy * fwn;
+            const uint32_t w = j.a.fw0 + ww;
+            draw_group(dec + rem, plane, nsteps, step0, g, w, j.a.fy0 + yy, j.a.key0, j.a.key1, Qt,
+                       force_columns(w, j.fx0, j.fx1, j.a.g.dim_x));
+        }
+    } else {
+        for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < n;
+             idx += (uint64_t)gridDim.x * 256) {
+            const uint32_t g = (uint32_t)(idx / plane);
+            const uint64_t rem = idx - g * plane;
+            const uint32_t yy = (uint32_t)(rem / fwn), ww = (uint32_t)(rem - (uint64_t)yy * fwn);
+            const uint32_t w = j.a.fw0 + ww;
+            draw_group(dec + rem, plane, nsteps, step0, g, w, j.a.fy0 + yy, j.a.key0, j.a.key1, Qt,
+                       force_columns(w, j.fx0, j.fx1, j.a.g.dim_x));
+        }
     }
     // fdec_step0 is not the field the other blocks read
     if (blockIdx.x == 0 && threadIdx.x == 0) state[slot].fdec_step0 = step0;

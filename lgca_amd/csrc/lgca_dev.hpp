@@ -188,21 +188,30 @@ __device__ __forceinline__ uint32_t fill_word(uint32_t w, uint32_t y, uint32_t d
     return word;
 }
 
-// Per-cell force decisions of word w of row y at `step`: 32 independent
-// Bernoulli(thr / 2^32) bits by first-fire sampling.  u0 is the word's first
-// uniform, philox({w, y, step >> 2, 1})[step & 3] (one call serves four
-// steps); further uniforms, needed only after a fire, come from
-// philox({w, y, step, 5 + 4i}).  Qt[k] = Q_{k+1}, where Q_1 = q = 2^32 - thr
+// Per-cell force decisions of word w of row y at one step of the four-step
+// group G = step >> 2: 32 independent Bernoulli(thr / 2^32) bits by
+// first-fire sampling.  u0 is the word's first uniform for this step,
+// philox({w, y, G, 1})[step & 3] (one call serves the group).  Further
+// uniforms, needed only after a fire, are the group's extras
+// E_e = philox({w, y, G, 5 + 4 (e >> 2)})[e & 3], e = 0, 1, ..., shared by the
+// group's steps in step order: `used` counts the extras the group's earlier
+// steps took and `ext` holds block used >> 2 (the caller loads block 0 when
+// any step of the group fires).  Qt[k] = Q_{k+1}, where Q_1 = q = 2^32 - thr
 // and Q_{k+1} = floor(Q_k q / 2^32) (force_table); the first of the next n
 // cells to fire is the smallest m < n with u + Q_{m+1} < 2^32, and there is
 // none when u + Q_n >= 2^32.
-__device__ __forceinline__ uint32_t force_word(uint32_t u0, uint32_t w, uint32_t y, uint32_t step,
-                                               uint32_t key0, uint32_t key1, const uint32_t* Qt)
+__device__ __forceinline__ bool force_fires(uint32_t u0, const uint32_t* Qt)
+{
+    return (uint64_t)u0 + Qt[31] < ((uint64_t)1 << 32);
+}
+
+__device__ __forceinline__ uint32_t force_word(uint32_t u0, uint32_t w, uint32_t y, uint32_t group,
+                                               uint32_t key0, uint32_t key1, const uint32_t* Qt,
+                                               uint32_t& used, U4& ext)
 {
     constexpr uint64_t kTwo32 = (uint64_t)1 << 32;
-    if ((uint64_t)u0 + Qt[31] >= kTwo32) return 0;  // no fire among all 32 cells
-    uint32_t word = 0, pos = 0, used = 0, u = u0;
-    U4 ext{0, 0, 0, 0};
+    if (!force_fires(u0, Qt)) return 0;  // no fire among all 32 cells
+    uint32_t word = 0, pos = 0, u = u0;
     for (;;) {
         const uint32_t n = 32 - pos;
         if ((uint64_t)u + Qt[n - 1] >= kTwo32) break;  // no fire among the next n cells
@@ -215,7 +224,7 @@ __device__ __forceinline__ uint32_t force_word(uint32_t u0, uint32_t w, uint32_t
         word |= 1u << (pos + lo);
         pos += lo + 1;
         if (pos == 32) break;
-        if ((used & 3u) == 0) ext = philox4x32_10(w, y, step, 5u + 4u * (used >> 2), key0, key1);
+        if ((used & 3u) == 0 && used != 0) ext = philox4x32_10(w, y, group, 5u + used, key0, key1);
         const uint32_t i = used & 3u;
         u = i == 0 ? ext.x : (i == 1 ? ext.y : (i == 2 ? ext.z : ext.w));
         ++used;
@@ -251,21 +260,34 @@ __device__ __forceinline__ uint32_t force_columns(uint32_t w, uint32_t fx0, uint
 // One thread's share of a force draw: the decision words of word w of row y
 // for the steps of group `group` (steps sharing (step & ~3), counted from
 // step0 & ~3) that fall in [step0, step0 + nsteps), stored at dec[k * plane]
-// for window index k; cols masks the forced columns of the word.
+// for window index k; cols masks the forced columns of the word.  Steps of
+// the group before step0 are walked too: they take extras before the later
+// steps do.
 __device__ __forceinline__ void draw_group(uint32_t* dec, uint64_t plane, uint32_t nsteps,
                                            uint32_t step0, uint32_t group, uint32_t w, uint32_t y,
                                            uint32_t key0, uint32_t key1, const uint32_t* Qt,
                                            uint32_t cols)
 {
     const uint32_t first = step0 & 3u;
-    const uint32_t tb = (step0 & ~3u) + 4u * group;  // first step of the group (mod 2^32)
-    const U4 r = philox4x32_10(w, y, tb >> 2, 1u, key0, key1);
-    const uint32_t u[4] = {r.x, r.y, r.z, r.w};
+    const uint32_t G = ((step0 & ~3u) + 4u * group) >> 2;  // of the group's steps mod 2^32
+    uint32_t u[4] = {0, 0, 0, 0};
+    U4 ext{0, 0, 0, 0};
+    if (cols) {
+        const U4 r = philox4x32_10(w, y, G, 1u, key0, key1);
+        u[0] = r.x; u[1] = r.y; u[2] = r.z; u[3] = r.w;
+        // one call for the first four extras of the whole group, made up front
+        // so that a wave whose lanes fire at different steps makes it once
+        if (force_fires(u[0], Qt) || force_fires(u[1], Qt) || force_fires(u[2], Qt) ||
+            force_fires(u[3], Qt))
+            ext = philox4x32_10(w, y, G, 5u, key0, key1);
+    }
+    uint32_t used = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
-        const uint32_t k = 4u * group + i - first;  // window index of step tb + i
-        if (4u * group + i < first || k >= nsteps) continue;
-        dec[(size_t)k * plane] = cols ? (force_word(u[i], w, y, tb + i, key0, key1, Qt) & cols) : 0u;
+        const uint32_t k = 4u * group + i - first;  // window index of step 4G + i
+        if (4u * group + i >= first + nsteps) break;
+        const uint32_t word = cols ? (force_word(u[i], w, y, G, key0, key1, Qt, used, ext) & cols) : 0u;
+        if (4u * group + i >= first) dec[(size_t)k * plane] = word;
     }
 }
 

@@ -222,25 +222,31 @@ hipError_t launch_step(int kind, const StepArgs& a, hipStream_t s)
 
 __global__ __launch_bounds__(256) void force_draw_kernel(ForceDrawArgs a)
 {
+    // grid: x over the (row, word) pairs of the force region, y over groups
     __shared__ uint32_t Qt[32];
     force_table(Qt, a.q);
-    const uint32_t rows = a.fy1 - a.fy0;
-    const uint32_t groups = ((a.step0 & 3u) + a.nsteps + 3u) / 4u;
+    const uint64_t plane = (uint64_t)(a.fy1 - a.fy0) * a.fwn;
     const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (uint64_t)groups * rows * a.fwn) return;
-    const uint32_t ww = (uint32_t)(idx % a.fwn);
-    const uint32_t yy = (uint32_t)((idx / a.fwn) % rows);
-    const uint32_t g = (uint32_t)(idx / ((uint64_t)a.fwn * rows));
+    if (idx >= plane) return;
+    uint32_t yy, ww;
+    if (plane < ((uint64_t)1 << 32)) {  // 32-bit division, the usual case
+        yy = (uint32_t)idx / a.fwn;
+        ww = (uint32_t)idx - yy * a.fwn;
+    } else {
+        yy = (uint32_t)(idx / a.fwn);
+        ww = (uint32_t)(idx - (uint64_t)yy * a.fwn);
+    }
     const uint32_t w = a.fw0 + ww;
-    draw_group(a.dec + (size_t)yy * a.fwn + ww, (uint64_t)rows * a.fwn, a.nsteps, a.step0, g, w,
-               a.fy0 + yy, a.key0, a.key1, Qt, force_columns(w, a.fx0, a.fx1, a.dim_x));
+    draw_group(a.dec + idx, plane, a.nsteps, a.step0, blockIdx.y, w, a.fy0 + yy, a.key0, a.key1, Qt,
+               force_columns(w, a.fx0, a.fx1, a.dim_x));
 }
 
 hipError_t launch_force_draw(const ForceDrawArgs& a, hipStream_t s)
 {
     const uint32_t groups = ((a.step0 & 3u) + a.nsteps + 3u) / 4u;
-    const uint64_t n = (uint64_t)groups * (a.fy1 - a.fy0) * a.fwn;
-    hipLaunchKernelGGL(force_draw_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, a);
+    const uint64_t n = (uint64_t)(a.fy1 - a.fy0) * a.fwn;
+    if ((n + 255) / 256 >= ((uint64_t)1 << 31) || groups > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(force_draw_kernel, dim3(div_up(n, 256), groups), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

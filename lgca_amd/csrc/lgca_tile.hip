@@ -59,13 +59,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint3
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
 }
+// `soff` is wave-uniform at every call; readfirstlane says so to the
+// compiler, which otherwise may reuse a vector copy of the value and wrap the
+// access in a loop over its distinct values.
 __device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, __builtin_amdgcn_readfirstlane(soff), 0);
 }
 __device__ __forceinline__ void bstore(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, __builtin_amdgcn_readfirstlane(soff), 0);
 }
 
 // Neighbouring words: the adjacent lane (DPP) when a wave holds consecutive
@@ -80,6 +83,15 @@ template <bool SEG>
 __device__ __forceinline__ uint32_t right_word(uint32_t v, const LaneCtx& c)
 {
     return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.rsrc, (int)v) : from_right(v);
+}
+
+// Output `sel` (wave-uniform) of a Philox call, by selects on the values
+// (a chain over an array would become a dynamically indexed stack array).
+__device__ __forceinline__ uint32_t pick4(const U4& r, uint32_t sel)
+{
+    const uint32_t lo = (sel & 1u) ? r.y : r.x;
+    const uint32_t hi = (sel & 1u) ? r.w : r.z;
+    return (sel & 2u) ? hi : lo;
 }
 
 // out cell x = in cell x-1, cyclically over dim_x
@@ -171,9 +183,17 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         if (SEG) return bload(make_rsrc(src, g.dim_y * row_b), r * row_b + c.wb, (uint32_t)d * plane_b);
         return bload(make_rsrc(src + (size_t)r * Q * g.P, row_b), c.wb, (uint32_t)d * plane_b);
     };
-    auto store_plane = [&](uint32_t r, int d, uint32_t v) {
-        if (SEG) bstore(v, make_rsrc(dst, g.dim_y * row_b), r * row_b + c.wb, (uint32_t)d * plane_b);
-        else bstore(v, make_rsrc(dst + (size_t)r * Q * g.P, row_b), c.wb, (uint32_t)d * plane_b);
+    // Stores and force-decision loads are issued by every lane: a lane that
+    // must not access memory gets the offset kOutside, past any descriptor's
+    // size, and the hardware drops its store or returns 0 for its load.  An
+    // exec-masked branch around them would make the count of memory
+    // operations issued after the next row's loads path-dependent, and the
+    // waits for those loads would then wait for the stores as well.
+    constexpr uint32_t kOutside = 0x80000000u;
+    const uint32_t fplane_b = (a.fy1 - a.fy0) * a.fwn * 4u;  // one step's force decisions
+    auto store_plane = [&](uint32_t r, int d, uint32_t v, bool wr) {
+        if (SEG) bstore(v, make_rsrc(dst, g.dim_y * row_b), wr ? r * row_b + c.wb : kOutside, (uint32_t)d * plane_b);
+        else bstore(v, make_rsrc(dst + (size_t)r * Q * g.P, row_b), wr ? c.wb : kOutside, (uint32_t)d * plane_b);
     };
     auto load_mask = [&](uint32_t r, int b) -> uint32_t {
         if (SEG) return bload(make_rsrc(a.mask, g.dim_y * 2u * plane_b), r * 2u * plane_b + c.wb, (uint32_t)b * plane_b);
@@ -211,6 +231,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     };
     const bool force_cols = a.fdec && wave_hits((int)a.fw0, (int)(a.fw0 + a.fwn));
     const bool force_mine = a.fdec && c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
+    const uint32_t fvoff = force_mine ? (c.w - a.fw0) * 4u : kOutside;  // strip mode
     // Decide once per tile whether the mask and force paths can be needed
     // (segment mode: whenever the lattice has masks / forcing; rows differ
     // per segment).
@@ -226,6 +247,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         }
     }
 
+    auto walk = [&](auto side_tag) {
+    constexpr bool SIDE = decltype(side_tag)::value;
+    const bool t_solids = SIDE && tile_solids, t_force = SIDE && tile_force;
     // pipeline windows between level b and b+1: rows j-1 (planes 1, 2) and j
     uint32_t winA[K][7];
     uint32_t winB[K][2];
@@ -256,7 +280,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     uint32_t nxt[7];
     uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
     {
-        if (tile_solids && (SEG || row_has_solids(row_i))) {
+        if (t_solids && (SEG || row_has_solids(row_i))) {
             nns = load_mask(row_i, 0);
             nsl = load_mask(row_i, 1);
         }
@@ -284,20 +308,29 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         if (i + 1 < n_iter) {
             const uint32_t rn = row_next;
             nns = nsl = 0;
-            if (tile_solids && (SEG || row_has_solids(rn))) {
+            if (t_solids && (SEG || row_has_solids(rn))) {
                 nns = load_mask(rn, 0);
                 nsl = load_mask(rn, 1);
             }
-            if (tile_force) {
+            if (t_force) {
+                // one buffer load per level on every lane, reading 0 where
+                // there is no decision
 #pragma unroll
                 for (int s = 1; s <= K; ++s) {
-                    fnext[s] = 0;
                     // next iteration, level s works on row_i - (s - 1)
                     const uint32_t jn = s == 1 ? row_i : lrow[s - 1];
-                    if ((STEADY || i + 1 >= (uint32_t)(2 * s)) && jn >= a.fy0 && jn < a.fy1 &&
-                        force_mine)
-                        fnext[s] = a.fdec[(size_t)(s - 1) * a.fdec_stride +
-                                          (size_t)(jn - a.fy0) * a.fwn + (c.w - a.fw0)];
+                    const bool lev = STEADY || i + 1 >= (uint32_t)(2 * s);
+                    const uint32_t* base = a.fdec + (size_t)(s - 1) * a.fdec_stride;
+                    if (SEG) {  // rows differ per lane
+                        const bool ok = lev && jn >= a.fy0 && jn < a.fy1 && force_mine;
+                        const uint32_t voff = ok ? ((jn - a.fy0) * a.fwn + (c.w - a.fw0)) * 4u : kOutside;
+                        fnext[s] = bload(make_rsrc(base, fplane_b), voff, 0);
+                    } else {    // the row is uniform: its offset is scalar, an
+                                // empty descriptor reads 0 for rows outside
+                        const bool ok = lev && jn >= a.fy0 && jn < a.fy1;
+                        fnext[s] = bload(make_rsrc(base, ok ? fplane_b : 0u), fvoff,
+                                         ok ? (jn - a.fy0) * a.fwn * 4u : 0u);
+                    }
                 }
             }
 #pragma unroll
@@ -348,9 +381,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                         // t0 even: level 1 uses word t0 & 3, level 2 the next one
                         if (s == 1) {
                             const U4 r = philox4x32_10(c.w, j, t >> 2, 0u, a.key0, a.key1);
-                            callw[1][0] = r.x; callw[1][1] = r.y; callw[1][2] = r.z; callw[1][3] = r.w;
-                            const uint32_t sel = t & 3u;
-                            p = sel == 0 ? r.x : (sel == 1 ? r.y : (sel == 2 ? r.z : r.w));
+                            p = pick4(r, t & 3u);
+                            // level 2 takes the next word next iteration
+                            if (K == 2) callw[1][0] = pick4(r, (t + 1u) & 3u);
                         } else {
                             p = dl[s][0];
                         }
@@ -363,14 +396,14 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                     }
                 }
                 collide<KIND, uint32_t>(st, p);
-                if (tile_solids) {
+                if (t_solids) {
                     const uint32_t ns = mns[s], sl = msl[s];
                     if (__builtin_amdgcn_ballot_w64((ns | sl) != 0) != 0)
                         apply_solids<KIND, uint32_t>(st, in, ns, sl, edge, j == 0 || j + 1 == g.dim_y);
                     // fcur is zero outside the forced rows
-                    if (tile_force && (SEG || (j >= a.fy0 && j < a.fy1)))
+                    if (t_force && (SEG || (j >= a.fy0 && j < a.fy1)))
                         apply_force<KIND, uint32_t>(st, fcur[s] & ~(ns | sl), a.force_dir);
-                } else if (tile_force && (SEG || (j >= a.fy0 && j < a.fy1))) {
+                } else if (t_force && (SEG || (j >= a.fy0 && j < a.fy1))) {
                     apply_force<KIND, uint32_t>(st, fcur[s], a.force_dir);
                 }
 #pragma unroll
@@ -391,15 +424,16 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
 #pragma unroll
                 for (int d = 0; d < Q; ++d) cur[d] = out[d];
                 // segment mode: rows past a segment's own tile belong to the next
-                if (s == K && writer && (!SEG || i - (uint32_t)(2 * K) < rows)) {
+                if (s == K) {
+                    const bool wr = writer && (!SEG || i - (uint32_t)(2 * K) < rows);
 #pragma unroll
-                    for (int d = 0; d < Q; ++d) store_plane(j, d, cur[d]);
+                    for (int d = 0; d < Q; ++d) store_plane(j, d, cur[d], wr);
                 }
             }
         }
 
         // mask delay line: level s next iteration handles the row level s-1 had
-        if (tile_solids) {
+        if (t_solids) {
 #pragma unroll
             for (int s = K; s >= 1; --s) {
                 mns[s] = mns[s - 1];
@@ -414,9 +448,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
 #pragma unroll
             for (int s = 2; s <= K; ++s) {
                 if (K <= 2) {
-                    const uint32_t sel = (a.t0 + 1u) & 3u;
-                    dl[s][0] = sel == 0 ? callw[1][0]
-                             : (sel == 1 ? callw[1][1] : (sel == 2 ? callw[1][2] : callw[1][3]));
+                    dl[s][0] = callw[1][0];
                 } else {
                     const int m = (s - 1) & 3;
                     if (m == 0) continue;
@@ -447,16 +479,54 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         iterate(T{}, P1{}, i + 1);
     }
     if (i < n_iter) iterate(T{}, P0{}, i);
+    };  // walk
+
+    // Most tiles of a large lattice are all fluid and unforced: they take a
+    // copy of the walk with that code left out (no tests, no branches).  Rows
+    // with a short last word keep the one walk: the second copy would push
+    // their kernels past the register budget.
+    if (RAGGED || tile_solids || tile_force) walk(std::true_type{});
+    else walk(std::false_type{});
 }
+
+// Strip-mode block order.  A block holds four consecutive strips of one row
+// tile; block columns are those four-strip groups.  Waves whose strips carry
+// forcing run longer, and dispatched in plain order the last of them trail
+// the rest of the grid; so the heavy block columns (of every row tile) are
+// dispatched first and the light ones fill in behind them.
+constexpr uint32_t kMaxHeavy = 8;
+struct TileOrder {
+    uint32_t nbx;               // block columns
+    uint32_t ntiles;            // row tiles
+    uint32_t nheavy;            // heavy block columns, dispatched first (0: plain order)
+    uint32_t heavy[kMaxHeavy];  // their indices, ascending
+};
 
 template <int KIND, int K, bool RAGGED, bool SEG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
-void tile_kernel(TileArgs a)
+void tile_kernel(TileArgs a, TileOrder o)
 {
-    // strip mode: grid (strips / 4, tiles); segment mode: grid (tile groups / 4)
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (SEG) tile_body<KIND, K, RAGGED, true>(a, 0, wave);
-    else tile_body<KIND, K, RAGGED, false>(a, wave, blockIdx.y);
+    // segment mode: grid (tile groups / 4); strip mode: grid (nbx * ntiles)
+    if (SEG) {
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+        tile_body<KIND, K, RAGGED, true>(a, 0, wave);
+        return;
+    }
+    const uint32_t L = blockIdx.x;
+    const uint32_t H = o.nheavy * o.ntiles;
+    uint32_t bx, tile;
+    if (L < H) {
+        tile = L / o.nheavy;
+        bx = o.heavy[L - tile * o.nheavy];
+    } else {
+        const uint32_t nl = o.nbx - o.nheavy, l = L - H;
+        tile = l / nl;
+        bx = l - tile * nl;  // the bx-th light column: step over the heavy ones
+        for (uint32_t h = 0; h < o.nheavy; ++h)
+            if (o.heavy[h] <= bx) ++bx;
+    }
+    const uint32_t strip = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
+    tile_body<KIND, K, RAGGED, false>(a, strip, tile);
 }
 
 // Batched form: every wave takes one work item (job, strip, tile, shape) and
@@ -550,16 +620,43 @@ hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s)
     if (a.th < 2 || (a.th & 1u) || a.th > 65535) return hipErrorInvalidValue;
     const uint32_t ntiles = (a.g.dim_y + a.th - 1) / a.th;
     dim3 grid;
+    TileOrder o{};
     if (seg) {
         const uint32_t groups = (ntiles + 64 / a.g.W - 1) / (64 / a.g.W);
         grid = dim3((groups + 3) / 4, 1);
     } else {
-        const uint32_t nstrips = (a.g.W + (64u - 2u * a.halo) - 1) / (64u - 2u * a.halo);
-        grid = dim3((nstrips + 3) / 4, ntiles);
+        const uint32_t stride = 64u - 2u * a.halo;
+        const uint32_t nstrips = (a.g.W + stride - 1) / stride;
+        o.nbx = (nstrips + 3) / 4;
+        o.ntiles = ntiles;
+        if ((uint64_t)o.nbx * ntiles >= ((uint64_t)1 << 31)) return hipErrorInvalidValue;
+        grid = dim3(o.nbx * ntiles);
+        // heavy block columns: those with a strip over a forced word; strip
+        // st spans words [st * stride - halo, + 64) modulo W
+        if (a.fdec && a.fwn && nstrips <= 4096) {
+            const int64_t W = a.g.W;
+            auto hits_force = [&](uint32_t st) {
+                if (W <= 64) return true;  // the strip wraps the whole row
+                const int64_t a0 = (((int64_t)st * stride - a.halo) % W + W) % W;
+                for (int64_t k = -1; k <= 1; ++k) {
+                    const int64_t b0 = (int64_t)a.fw0 + k * W, b1 = b0 + a.fwn;
+                    if (a0 < b1 && a0 + 64 > b0) return true;
+                }
+                return false;
+            };
+            for (uint32_t bx = 0; bx < o.nbx; ++bx) {
+                bool heavy = false;
+                for (uint32_t st = 4 * bx; st < 4 * bx + 4 && st < nstrips; ++st) heavy |= hits_force(st);
+                if (!heavy) continue;
+                if (o.nheavy == kMaxHeavy) { o.nheavy = 0; break; }  // many: plain order
+                o.heavy[o.nheavy++] = bx;
+            }
+            if (o.nheavy == o.nbx) o.nheavy = 0;
+        }
     }
     const bool ragged = a.g.nb != 32;
 #define LGCA_TILE_LAUNCH(KD, KK, RG, SG) \
-    hipLaunchKernelGGL((tile_kernel<KD, KK, RG, SG>), grid, dim3(256), 0, s, a)
+    hipLaunchKernelGGL((tile_kernel<KD, KK, RG, SG>), grid, dim3(256), 0, s, a, o)
 #define LGCA_TILE_CASE(KD, KK)                                                                 \
     if (kind == KD && k == KK) {                                                             \
         if (ragged) {                                                                        \

@@ -96,15 +96,13 @@ static void force_cdf(uint64_t thr, uint64_t C[32])
     }
 }
 
-static uint32_t force_word_cdf(const uint32_t key[2], const uint64_t C[32], uint64_t step,
-                               uint32_t y, uint32_t w)
+/* The decision word of one step, walking from uniform u; *used counts the
+ * group's extras taken so far and ext[] holds the current block of four. */
+static uint32_t force_walk(const uint32_t key[2], const uint64_t C[32], uint32_t group, uint32_t y,
+                           uint32_t w, uint32_t u, unsigned* used, uint32_t ext[4])
 {
-    uint32_t prim[4], ext[4];
-    const uint32_t c0[4] = { w, y, (uint32_t)step >> 2, 1u };
-    oracle_philox4x32_10(c0, key, prim);
-    uint32_t u = prim[step & 3];
     uint32_t word = 0;
-    unsigned pos = 0, used = 0;  /* used: extra uniforms taken so far */
+    unsigned pos = 0;
     for (;;) {
         const unsigned n = 32 - pos;
         if ((uint64_t)u >= C[n - 1]) break;       /* no fire among the next n cells */
@@ -113,13 +111,29 @@ static uint32_t force_word_cdf(const uint32_t key[2], const uint64_t C[32], uint
         word |= 1u << (pos + m);
         pos += m + 1;
         if (pos == 32) break;
-        if ((used & 3) == 0) {
-            const uint32_t c1[4] = { w, y, (uint32_t)step, 5u + 4u * (used >> 2) };
+        if ((*used & 3) == 0) {
+            const uint32_t c1[4] = { w, y, group, 5u + 4u * (*used >> 2) };
             oracle_philox4x32_10(c1, key, ext);
         }
-        u = ext[used & 3];
-        ++used;
+        u = ext[*used & 3];
+        ++*used;
     }
+    return word;
+}
+
+/* Walks the steps of the word's four-step group up to `step`: they share the
+ * group's extras in step order. */
+static uint32_t force_word_cdf(const uint32_t key[2], const uint64_t C[32], uint64_t step,
+                               uint32_t y, uint32_t w)
+{
+    uint32_t prim[4], ext[4] = { 0, 0, 0, 0 };
+    const uint32_t group = (uint32_t)step >> 2;
+    const uint32_t c0[4] = { w, y, group, 1u };
+    oracle_philox4x32_10(c0, key, prim);
+    unsigned used = 0;
+    uint32_t word = 0;
+    for (unsigned i = 0; i <= (unsigned)(step & 3); ++i)
+        word = force_walk(key, C, group, y, w, prim[i], &used, ext);
     return word;
 }
 

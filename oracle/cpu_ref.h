@@ -23,9 +23,13 @@
  *   forcing word (y, w)     = 32 independent Bernoulli(thr / 2^32) bits drawn
  *                             by first-fire sampling (oracle_force_word):
  *                             uniform 0 is philox({w, y, step>>2, 1})[step & 3]
- *                             (one call per word and four steps), uniform
- *                             j >= 1 is philox({w, y, step, 5 + 4((j-1)>>2)})
- *                             [(j-1) & 3]; with q = 2^32 - thr, Q_1 = q and
+ *                             (one call per word and four steps); later
+ *                             uniforms are the extras of the four-step group
+ *                             G = step>>2, E_e = philox({w, y, G, 5 + 4(e>>2)})
+ *                             [e & 3], e = 0, 1, ..., taken in order by the
+ *                             group's steps in step order (a step starts after
+ *                             the extras its group's earlier steps took);
+ *                             with q = 2^32 - thr, Q_1 = q and
  *                             Q_{k+1} = floor(Q_k q / 2^32), the first of the
  *                             next n cells to fire is the smallest m < n with
  *                             u < 2^32 - Q_{m+1} (none when u >= 2^32 - Q_n);

@@ -91,6 +91,10 @@ STEP_CASES = [
     (3, 8232, 6, "mixed", dict(dir="x", p=0.02, x0=8100)),  # two workgroups per row
     (2, 4096, 8, "periodic", dict(dir="x", p=0.3, x0=0, x1=40)),  # force on the wrapped words
     (3, 1000, 66, "cylinder", dict(dir="x", p=0.01, x0=0, x1=64)),
+    # forced strips in some block columns only: those columns are dispatched
+    # first (the first of two; the middle of three)
+    (2, 8192, 24, "cylinder", dict(dir="x", p=0.3, x0=3000, x1=3100)),
+    (3, 16384, 20, "mixed", dict(dir="y", p=0.2, x0=9000, x1=9100)),
     # tall narrow rows: many row tiles per wave (segment mode), idle lanes
     (2, 96, 300, "mixed", dict(dir="x", p=0.1, y0=50, y1=250)),
     (0, 700, 50, "cylinder", None),

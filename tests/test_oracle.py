@@ -81,28 +81,38 @@ def test_force_word_half_probability_closed_form(orc):
     """At p = 1/2 the first-fire thresholds are 2^32 - 2^(31-m) exactly, so the
     first forced cell among the next n is at offset (leading one-bits of the
     uniform), none when that is >= n.  This pins the counter layout (which
-    Philox outputs, in which order) and the walk, independently of the
-    integer recurrence."""
+    Philox outputs, in which order), the sharing of a group's extras by its
+    four steps in step order, and the walk, independently of the integer
+    recurrence.  At p = 1/2 nearly every step takes extras, so a later step of
+    a group reading the group's extras from the start would fail here."""
     L = orc.lib()
     seed = 0xFEEDFACE12345678
     key = [seed & 0xFFFFFFFF, seed >> 32]
     thr = orc.threshold(0.5)
     assert thr == 2 ** 31
-    for (y, w, step) in [(0, 0, 0), (1, 2, 3), (77, 5, 4999), (8191, 1023, 2 ** 33 + 7)]:
-        st = step & 0xFFFFFFFF
-        us = [orc.philox([w, y, st >> 2, 1], key)[step & 3]]
-        for j in range(32):
-            us.append(orc.philox([w, y, st, 5 + 4 * (j >> 2)], key)[j & 3])
-        want, pos, i = 0, 0, 0
-        while True:
-            m = _leading_ones(us[i])
-            i += 1
-            if m >= 32 - pos:
-                break
-            want |= 1 << (pos + m)
-            pos += m + 1
-            if pos == 32:
-                break
+    cases = [(0, 0, 0), (1, 2, 3), (77, 5, 4999), (8191, 1023, 2 ** 33 + 7),
+             (4, 9, 4), (4, 9, 5), (4, 9, 6), (4, 9, 7)]
+    for (y, w, step) in cases:
+        group = (step & 0xFFFFFFFF) >> 2
+        prim = orc.philox([w, y, group, 1], key)
+        extras = []
+        for e in range(4 * 32):
+            extras.append(orc.philox([w, y, group, 5 + 4 * (e >> 2)], key)[e & 3])
+        used = 0
+        for i in range((step & 3) + 1):  # the group's steps up to this one
+            us = [prim[i]]
+            want, pos, k = 0, 0, 0
+            while True:
+                m = _leading_ones(us[k])
+                k += 1
+                if m >= 32 - pos:
+                    break
+                want |= 1 << (pos + m)
+                pos += m + 1
+                if pos == 32:
+                    break
+                us.append(extras[used])
+                used += 1
         assert L.oracle_force_word(seed, step, y, w, thr) == want, (y, w, step)
 
 
