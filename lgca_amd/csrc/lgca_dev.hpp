@@ -205,24 +205,33 @@ __device__ __forceinline__ bool force_fires(uint32_t u0, const uint32_t* Qt)
     return (uint64_t)u0 + Qt[31] < ((uint64_t)1 << 32);
 }
 
+// Offset of the first fire among 32 cells for uniform u, 32 when none fires:
+// the number of k in [0, 32) with u + Qt[k] >= 2^32, a prefix because Qt
+// decreases.  Five fixed lifting steps find it up to 31, and Qt[31] adds the
+// last; no branches, so the lanes of a wave stay in step.  For the next n
+// cells of a word the first fire is this offset when it is < n, else there is
+// none (the prefix covers the first n cells).
+__device__ __forceinline__ uint32_t force_first(uint32_t u, const uint32_t* Qt)
+{
+    constexpr uint64_t kTwo32 = (uint64_t)1 << 32;
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t step = 16; step; step >>= 1)
+        m += ((uint64_t)u + Qt[m + step - 1] >= kTwo32) ? step : 0u;
+    return m + (((uint64_t)u + Qt[31] >= kTwo32) ? 1u : 0u);
+}
+
 __device__ __forceinline__ uint32_t force_word(uint32_t u0, uint32_t w, uint32_t y, uint32_t group,
                                                uint32_t key0, uint32_t key1, const uint32_t* Qt,
                                                uint32_t& used, U4& ext)
 {
-    constexpr uint64_t kTwo32 = (uint64_t)1 << 32;
     if (!force_fires(u0, Qt)) return 0;  // no fire among all 32 cells
     uint32_t word = 0, pos = 0, u = u0;
     for (;;) {
-        const uint32_t n = 32 - pos;
-        if ((uint64_t)u + Qt[n - 1] >= kTwo32) break;  // no fire among the next n cells
-        uint32_t lo = 0, hi = n - 1;                    // the condition is monotone in m
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if ((uint64_t)u + Qt[mid] < kTwo32) hi = mid;
-            else lo = mid + 1;
-        }
-        word |= 1u << (pos + lo);
-        pos += lo + 1;
+        const uint32_t m = force_first(u, Qt);
+        if (m >= 32 - pos) break;  // no fire among the word's remaining cells
+        word |= 1u << (pos + m);
+        pos += m + 1;
         if (pos == 32) break;
         if ((used & 3u) == 0 && used != 0) ext = philox4x32_10(w, y, group, 5u + used, key0, key1);
         const uint32_t i = used & 3u;
