@@ -27,7 +27,7 @@ class Info(C.Structure):
         ("model", C.c_int32), ("num_dir", C.c_uint32), ("dim_x", C.c_uint32),
         ("dim_y", C.c_uint32), ("coarse_radius", C.c_uint32), ("coarse_dim_x", C.c_uint32),
         ("coarse_dim_y", C.c_uint32), ("words_per_row", C.c_uint32), ("row_pitch", C.c_uint32),
-        ("reserved", C.c_uint32), ("num_cells", C.c_uint64), ("num_coarse_cells", C.c_uint64),
+        ("tile_rows", C.c_uint32), ("num_cells", C.c_uint64), ("num_coarse_cells", C.c_uint64),
         ("state_bytes", C.c_uint64), ("device_bytes", C.c_uint64), ("step", C.c_uint64),
         ("seed", C.c_uint64),
     ]

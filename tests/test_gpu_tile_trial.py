@@ -1,0 +1,78 @@
+"""The timed trial of tile heights on large lattices (lgca_set_tuning with
+automatic rows): it settles on one of its candidates, never changes results,
+and restarts when the workload changes."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+# the fixed rule's own height (8 rows on this lattice) and the trial's others
+CANDIDATES = {8, 12, 16, 20, 24, 32}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_lattice(dx, dy, seed):
+    from lgca_amd import Lattice
+    rng = np.random.default_rng(seed)
+    types = np.zeros((dy, dx), dtype=np.uint8)
+    types[0, :] = types[-1, :] = 1
+    ys, xs = np.mgrid[0:dy, 0:dx]
+    types[(xs - dx // 3) ** 2 + (ys - dy // 2) ** 2 < (dy // 5) ** 2] = 1
+    state = rng.integers(0, 128, dx * dy, dtype=np.uint8)
+    lat = Lattice(2, dim_x=dx, dim_y=dy, coarse_graining_radius=1, seed=seed)
+    lat.set_cell_types(types.reshape(-1))
+    lat.set_state_u8(state)
+    lat.set_forcing("x", 0.02, 0, 200)
+    return lat, types.reshape(-1), state
+
+
+def test_trial_picks_a_candidate_and_keeps_results():
+    dx, dy, seed = 4096, 1024, 77  # 2^22 cells: the smallest lattice with a trial
+    say(f"[trial] {dx}x{dy} FHP-II, cylinder, forced strip")
+    lat, types, state = make_lattice(dx, dy, seed)
+    ref, _, _ = make_lattice(dx, dy, seed)
+    ref.set_tuning(4, 20)
+    assert lat.info().tile_rows == 0
+    # 6 candidates x 2 launches x 4 steps = 48 steps of trial, in uneven calls
+    for n in (5, 3, 17, 9, 30, 1, 22, 13):
+        lat.step(n)
+        ref.step(n)
+    lat.synchronize()
+    lat.step(4)  # decides now that the trial's launches have finished
+    ref.step(4)
+    rows = lat.info().tile_rows
+    say(f"[trial] chosen tile rows {rows}")
+    assert rows in CANDIDATES
+    np.testing.assert_array_equal(lat.get_state_u8(), ref.get_state_u8())
+    assert lat.steps == ref.steps == 104
+    # and both agree with the oracle
+    want = orc.run(2, dx, dy, types, state, seed, 0, 104,
+                   dict(dir="x", p=0.02, x0=0, x1=200), nthreads=8)
+    np.testing.assert_array_equal(lat.get_state_u8(), want)
+    # a new forcing restarts the trial; an explicit height is reported as set
+    lat.set_forcing("x", 0.01, 0, 100)
+    assert lat.info().tile_rows == 0
+    lat.set_tuning(4, 18)
+    assert lat.info().tile_rows == 18
+    lat.close()
+    ref.close()
+
+
+def test_small_lattices_keep_the_rule():
+    from lgca_amd import Lattice
+    lat = Lattice(2, dim_x=1024, dim_y=512, coarse_graining_radius=1, seed=3)
+    lat.apply_bc_periodic()
+    lat.init_random(0.3)
+    lat.step(80)
+    lat.synchronize()
+    lat.step(4)
+    assert lat.info().tile_rows == 0
+    lat.close()
