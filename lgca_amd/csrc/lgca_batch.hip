@@ -4,8 +4,9 @@
 // A batch keeps every admitted job's bit-planes resident in one device arena
 // and advances all of them together.  A round gives each running job k steps
 // (k = the tuned steps per pass, less at a job's end) and is one launch of
-// tile_batch_kernel per (model, k, ragged width) class, followed by
-// advance_kernel, which moves every job's step and buffer parity on the device.
+// tile_batch_kernel per (model, k, ragged width) class; the wave that finishes
+// a job's last work item of the round moves the job's step and buffer parity
+// on the device.
 // Work items (one per wave: job, x-strip, row tile, tile shape) are rebuilt
 // only when the set of running jobs or the tuning changes, so a steady stretch
 // of rounds is launches alone.  Per-cell force decisions are drawn for windows
@@ -426,8 +427,6 @@ struct lgca_batch {
     int staging_half = 0;
     hipEvent_t staging_ev[2] = {nullptr, nullptr};
     std::vector<ClassRange> classes;
-    uint32_t n_entries = 0;
-    const uint32_t* d_entries = nullptr;
     std::vector<uint64_t> sig, last_sig;
     std::deque<hipEvent_t> inflight;
     std::vector<hipEvent_t> event_pool;
@@ -804,7 +803,7 @@ uint32_t tile_rows_for(uint64_t wave_rows_total, uint32_t waves)
     return (uint32_t)th;
 }
 
-// Builds this round's work items and advance entries if the mix changed.
+// Builds this round's work items if the mix changed.
 void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
 {
     const size_t n = b->active.size();
@@ -815,8 +814,6 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     b->last_sig = b->sig;
     // classes in a fixed order: kind, k, ragged (segment and strip jobs share)
     std::map<uint64_t, std::vector<WorkItem>> cls;
-    std::vector<uint32_t> entries;
-    entries.reserve(n);
     for (size_t i = 0; i < n; ++i) {
         const Job* j = b->active[i];
         const Geometry& g = j->p.g;
@@ -828,20 +825,19 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) | (g.nb != 32 ? 1u : 0u);
         std::vector<WorkItem>& v = cls[key];
         if (seg) {
-            const uint32_t S = 64 / g.W;
-            for (uint32_t t = 0; t < (tiles + S - 1) / S; ++t)
-                v.push_back(WorkItem{(uint32_t)j->slot, 0u, t, jth | (1u << 31)});
+            const uint32_t S = 64 / g.W, groups = (tiles + S - 1) / S;
+            for (uint32_t t = 0; t < groups; ++t)
+                v.push_back(WorkItem{(uint32_t)j->slot, 0u, t, jth | (1u << 31), groups});
         } else {
             const uint32_t strips = tile_strips(g, k);
             for (uint32_t t = 0; t < tiles; ++t)
                 for (uint32_t s = 0; s < strips; ++s)
-                    v.push_back(WorkItem{(uint32_t)j->slot, s, t, jth | (halo << 16)});
+                    v.push_back(WorkItem{(uint32_t)j->slot, s, t, jth | (halo << 16), tiles * strips});
         }
-        entries.push_back((uint32_t)j->slot | ((uint32_t)k << 24));
     }
     size_t total = 0;
     for (auto& kv : cls) total += kv.second.size();
-    const size_t bytes = total * sizeof(WorkItem) + entries.size() * 4 + 16;
+    const size_t bytes = total * sizeof(WorkItem) + 16;
     const int ib = b->items_buf ^= 1;
     // at least 16 MiB, so a growing mix rarely drains the stream to grow it
     grow_buffer(b, (char**)&b->d_items[ib], &b->items_cap[ib], std::max<size_t>(bytes, (size_t)16 << 20),
@@ -860,10 +856,6 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         all.insert(all.end(), kv.second.begin(), kv.second.end());
     }
     upload(b, b->d_items[ib], all.data(), all.size() * sizeof(WorkItem));
-    uint32_t* dent = (uint32_t*)(b->d_items[ib] + all.size());
-    upload(b, dent, entries.data(), entries.size() * 4);
-    b->d_entries = dent;
-    b->n_entries = (uint32_t)entries.size();
 }
 
 // New force windows for forced jobs whose window ends before step + k, and
@@ -1013,7 +1005,6 @@ void run_round(lgca_batch* b)
                "tile_batch_kernel");
         ++nl;
     }
-    hipchk(launch_advance(b->d_state, b->d_entries, b->n_entries, b->stream), "advance_kernel");
     if (b->tuning) {
         b->tune_su[b->cand_i] += su;
         if (++b->cand_rounds == kTuneRounds) {

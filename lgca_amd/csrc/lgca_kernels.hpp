@@ -80,19 +80,19 @@ struct BatchState {
     uint32_t step;         // absolute step (mod 2^32)
     uint32_t cur;          // 0: state in buf0, 1: in buf1
     uint32_t fdec_step0;   // step of the first decision plane at a.fdec
-    uint32_t pad;
+    uint32_t done;         // waves of the current round that have finished
 };
 struct WorkItem {
     uint32_t job, strip, tile;  // segment mode: strip 0, tile = group of 64 / W row tiles
     uint32_t shape;        // rows per tile | halo << 16 | segment mode << 31
+    uint32_t job_items;    // items of this job in the round
 };
 // One class launch of a round: every item (all of one model, K and raggedness)
-// advances its job K steps reading its BatchState.  launch_advance then moves
-// each listed job (entry = slot | k << 24) k steps on and swaps its buffers.
-hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
-                             const BatchState* state, const WorkItem* items, uint32_t n_items,
-                             hipStream_t s);
-hipError_t launch_advance(BatchState* state, const uint32_t* entries, uint32_t n, hipStream_t s);
+// advances its job K steps reading its BatchState.  The wave that finishes a
+// job's last item of the round moves the job K steps on and swaps its buffers,
+// for the next round's launches to read.
+hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
+                             const WorkItem* items, uint32_t n_items, hipStream_t s);
 
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]

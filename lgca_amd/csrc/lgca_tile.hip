@@ -531,13 +531,13 @@ void tile_kernel(TileArgs a, TileOrder o)
 
 // Batched form: every wave takes one work item (job, strip, tile, shape) and
 // runs that job's lattice.  A job's step, current buffer and force-window
-// origin live in device memory (advance_kernel moves them after each round),
-// so queued rounds need no host-to-device traffic.  One launch covers the
+// origin live in device memory (the job's last wave of a round moves them on),
+// so queued rounds need no host-to-device traffic and no launch between them.  One launch covers the
 // items of one (model, K, ragged) class; the top bit of an item's shape picks
 // segment mode for its job, so strip and segment jobs share the launch.
 template <int KIND, int K, bool RAGGED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
-void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __restrict__ state,
+void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
                        const WorkItem* __restrict__ items, uint32_t n_items)
 {
     // wave-uniform by construction; readfirstlane lets the compiler keep the
@@ -555,22 +555,22 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, const BatchState* __re
     if (a.fdec) a.fdec += (size_t)(bs.step - bs.fdec_step0) * a.fdec_stride;
     if (it.shape >> 31) tile_body<KIND, K, RAGGED, true>(a, it.strip, it.tile);
     else tile_body<KIND, K, RAGGED, false>(a, it.strip, it.tile);
+    // The job's last wave of the round advances it.  Every other wave of the
+    // job has read the state and counted itself, so nothing in this launch
+    // reads the state after the update, and the next launch on the stream
+    // sees it.
+    if ((threadIdx.x & 63u) == 0) {
+        BatchState& st = state[it.job];
+        if (atomicAdd(&st.done, 1u) + 1u == it.job_items) {
+            st.done = 0;
+            st.step = bs.step + (uint32_t)K;
+            st.cur = bs.cur ^ 1u;
+        }
+    }
 }
 
-__global__ __launch_bounds__(256) void advance_kernel(BatchState* state, const uint32_t* entries,
-                                                      uint32_t n)
-{
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t e = entries[i];
-    BatchState& s = state[e & 0xFFFFFFu];
-    s.step += e >> 24;
-    s.cur ^= 1u;
-}
-
-hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
-                             const BatchState* state, const WorkItem* items, uint32_t n_items,
-                             hipStream_t s)
+hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
+                             const WorkItem* items, uint32_t n_items, hipStream_t s)
 {
     if (!n_items) return hipSuccess;
     const dim3 grid((n_items + 3) / 4);
@@ -587,13 +587,6 @@ hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs,
     LGCA_BATCH_CASE(kFHP2, 1) LGCA_BATCH_CASE(kFHP2, 2) LGCA_BATCH_CASE(kFHP2, 4) LGCA_BATCH_CASE(kFHP2, 8)
 #undef LGCA_BATCH_CASE
     return hipErrorInvalidValue;
-}
-
-hipError_t launch_advance(BatchState* state, const uint32_t* entries, uint32_t n, hipStream_t s)
-{
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(advance_kernel, dim3((n + 255) / 256), dim3(256), 0, s, state, entries, n);
-    return hipGetLastError();
 }
 
 uint32_t tile_halo(const Geometry& g, int k)
