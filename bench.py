@@ -330,7 +330,9 @@ def main() -> int:
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": w["label"], "cells": n, "directions": Q[w["model"]],
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "steps_per_pass": 4 if args.steps_per_pass is None else args.steps_per_pass,
+                   "tile_rows": lat.info().tile_rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
