@@ -61,8 +61,7 @@ typedef struct lgca_info {
     uint32_t tile_rows;      /* rows per tile of the tiled kernel's steps_per_pass
                                 launches: lgca_set_tuning's value, else the height
                                 the timed trial of the first launches chose (0
-                                until it has; lattices under 2^22 cells keep the
-                                fixed rule and report 0)                        */
+                                until it has)                                   */
     uint64_t num_cells;
     uint64_t num_coarse_cells;
     uint64_t state_bytes;    /* one state buffer: dim_y * Q * row_pitch * 4   */
@@ -135,10 +134,9 @@ LGCA_API lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps);
 
 /* Launch shape of lgca_step: steps fused per pass over HBM (1, 2, 4 or 8; 0
  * selects the one-step kernel that keeps a 128-cell quad per thread) and rows
- * per tile (0 = automatic: from the lattice size, and on lattices of 2^22 cells
- * or more by a timed trial of a few heights on the first launches, kept
- * thereafter and restarted by this call, lgca_set_forcing, new cell types or a
- * new stream).  Results do not depend on it.  No counterpart in the
+ * per tile (0 = automatic: a timed trial of a few heights on the first
+ * launches, kept thereafter and restarted by this call, lgca_set_forcing, new
+ * cell types or a new stream).  Results do not depend on it.  No counterpart in the
  * reference; used by the autotuner and benchmarks. */
 LGCA_API lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows);
 

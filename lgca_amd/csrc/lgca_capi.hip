@@ -676,9 +676,11 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
     const uint64_t wave_rows = tile_segments(h->g, k) ? (uint64_t)h->dim_y / (64 / h->g.W)
                                                       : (uint64_t)h->dim_y * tile_strips(h->g, k);
     // about 1.75 rounds of resident waves (four per SIMD on 1024 SIMDs); on the
-    // 32768 x 8192 lattice this gives 20-row tiles, the fastest measured
+    // 32768 x 8192 lattice this gives 20-row tiles, the fastest measured.  Small
+    // lattices get short tiles: a wave's walk of th + 2K rows is then their
+    // critical path (2-row tiles run the 2048 x 512 pipe 1.6x faster than 8)
     uint64_t th = wave_rows / 7168;
-    if (th < 8) th = 8;
+    if (th < 2) th = 2;
     if (th > 256) th = 256;
     if (th > h->dim_y) th = h->dim_y;
     th += th & 1u;  // tiles start on even rows
@@ -688,7 +690,7 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
 // Rows per tile for one k-step launch of the tiled kernel, and the trial slot
 // to time it in (-1: none).  With automatic tile rows, the first launches at
 // the configured steps per pass each run a candidate height (the rule's, then
-// 12, 16, 20, 24, 32 rows, two launches each) between two events on the
+// 4, 8, 12, 16, 20, 24, 32 rows, two launches each) between two events on the
 // lattice's stream; once the last has finished (queried, never waited for)
 // the fastest height is kept, the rule's unless another beats it by more
 // than 2 %.  The rule is fixed on the 32768 x 8192 lattice, where its 20 rows
@@ -699,14 +701,14 @@ static lgca_status tile_rows_for_launch(lgca_lattice* h, int k, uint32_t* th, in
 {
     *slot = -1;
     *th = auto_tile_rows(h, k);
-    if (h->tile_th || k != h->tile_k || (uint64_t)h->dim_x * h->dim_y < (1ull << 22)) return LGCA_OK;
+    if (h->tile_th || k != h->tile_k) return LGCA_OK;
     if (h->tuned_th) {
         *th = h->tuned_th;
         return LGCA_OK;
     }
     if (h->trial_th.empty() || h->trial_k != k) {
         h->trial_th.assign(1, *th);
-        for (uint32_t c : {12u, 16u, 20u, 24u, 32u})
+        for (uint32_t c : {4u, 8u, 12u, 16u, 20u, 24u, 32u})
             if (c <= h->dim_y && std::find(h->trial_th.begin(), h->trial_th.end(), c) == h->trial_th.end())
                 h->trial_th.push_back(c);
         h->trial_k = k;
