@@ -167,14 +167,14 @@ public:
     void init_random()
     {
         sync_types();
-        check(lgca_init_random(m_h, 1.0 / NUM_DIR));
+        check(lgca_init_random(dev(), 1.0 / NUM_DIR));
     }
     void init_single(const std::vector<size_t> occupied_nodes)
     {
         std::vector<uint8_t> s(m_num_cells);
-        check(lgca_get_state_u8(m_h, s.data()));
+        check(lgca_get_state_u8(dev(), s.data()));
         for (size_t n : occupied_nodes) s[n / 8] |= (uint8_t)(1u << (n % 8));
-        check(lgca_set_state_u8(m_h, s.data()));
+        check(lgca_set_state_u8(dev(), s.data()));
     }
     void init_single_collision()
     {
@@ -189,13 +189,13 @@ public:
         std::vector<uint8_t> region(m_num_cells);
         for (size_t c = 0; c < m_num_cells; ++c)
             region[c] = in_disc(c, (int)(m_dim_x / 2), (int)(m_dim_y / 2), (Real)(m_dim_y / 4)) ? 1 : 0;
-        check(lgca_init_random_masked(m_h, 1.0 / NUM_DIR, region.data()));
+        check(lgca_init_random_masked(dev(), 1.0 / NUM_DIR, region.data()));
     }
 
     unsigned long get_n_particles()
     {
         uint64_t n = 0;
-        check(lgca_count_particles(m_h, &n));
+        check(lgca_count_particles(dev(), &n));
         m_num_particles = n;
         return (unsigned long)n;
     }
@@ -220,11 +220,16 @@ public:
     // --------------------------------------------------------- compute path
     void setup_parallel() { std::printf("HIP configuration: one stream on GPU %d.\n\n", m_device); }
 
+    // One step, as OMP_Lattice::collide_and_propagate.  Steps are counted and
+    // enqueued together, as one lgca_step of up to kStepBatch steps, when
+    // anything else touches the lattice: the tiled kernel then advances several
+    // steps per pass over HBM instead of one (the viewers call this once per
+    // step).  Results are those of single steps.
     void collide_and_propagate(const bool p = false)
     {
         (void)p;
         sync_types();
-        check(lgca_step(m_h, 1));
+        if (++m_pending == kStepBatch) flush();
     }
 
     // Count-mode body force in direction m_bf_dir (src/omp_lattice.cpp:254-346).
@@ -233,37 +238,37 @@ public:
         if (m_bf_dir != 'x' && m_bf_dir != 'y') return;
         sync_types();
         int64_t reverted = 0;
-        check(lgca_apply_body_force_count(m_h, m_bf_dir == 'x' ? LGCA_FORCE_X : LGCA_FORCE_Y,
+        check(lgca_apply_body_force_count(dev(), m_bf_dir == 'x' ? LGCA_FORCE_X : LGCA_FORCE_Y,
                                           forcing, &reverted));
     }
 
     std::vector<Real> get_mean_velocity()
     {
         float v[2] = {0, 0};
-        check(lgca_mean_velocity(m_h, v));
+        check(lgca_mean_velocity(dev(), v));
         return std::vector<Real>{v[0], v[1]};
     }
 
     void post_process()
     {
-        check(lgca_post_process(m_h, m_cell_density_cpu, m_cell_momentum_cpu, m_mean_density_cpu,
+        check(lgca_post_process(dev(), m_cell_density_cpu, m_cell_momentum_cpu, m_mean_density_cpu,
                                 m_mean_momentum_cpu));
     }
 
     void copy_data_to_device() { sync_types(); }
-    void copy_data_from_device() { check(lgca_synchronize(m_h)); }
-    void copy_data_to_output_buffer() { check(lgca_snapshot(m_h)); }
+    void copy_data_from_device() { check(lgca_synchronize(dev())); }
+    void copy_data_to_output_buffer() { check(lgca_snapshot(dev())); }
 
     // ---------------------------------------------------------------- extras
-    void set_seed(uint64_t seed) { check(lgca_set_seed(m_h, seed)); }
+    void set_seed(uint64_t seed) { check(lgca_set_seed(dev(), seed)); }
     void step(uint64_t nsteps)
     {
         sync_types();
-        check(lgca_step(m_h, nsteps));
+        check(lgca_step(dev(), nsteps));
     }
     void set_forcing(char dir, double p, uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1)
     {
-        check(lgca_set_forcing(m_h, dir == 'x' ? LGCA_FORCE_X : (dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_NONE),
+        check(lgca_set_forcing(dev(), dir == 'x' ? LGCA_FORCE_X : (dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_NONE),
                                p, x0, x1, y0, y1));
     }
     // The viewer's forcing controller on the GPU: a tick decides on the
@@ -271,24 +276,25 @@ public:
     // steps and snapshots, without the host waiting (lgca_controller_*).
     void set_forcing_controller(int64_t forcing, int64_t equilibrium)
     {
-        check(lgca_controller_set(m_h, m_bf_dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_X, m_u, forcing,
+        check(lgca_controller_set(dev(), m_bf_dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_X, m_u, forcing,
                                   equilibrium));
     }
     void controlled_tick(uint64_t steps)
     {
         sync_types();
-        check(lgca_controller_tick(m_h, steps));
+        check(lgca_controller_tick(dev(), steps));
     }
     // Checkpoint of state, types, seed, step and forcing; resume with
     // lgca_load_checkpoint (include/lgca.h).
     void save_checkpoint(const std::string& path)
     {
         sync_types();
-        check(lgca_save_checkpoint(m_h, path.c_str()));
+        check(lgca_save_checkpoint(dev(), path.c_str()));
     }
-    void get_state_u8(uint8_t* out) { check(lgca_get_state_u8(m_h, out)); }
-    void set_state_u8(const uint8_t* in) { check(lgca_set_state_u8(m_h, in)); }
-    lgca_lattice* handle() { return m_h; }
+    void get_state_u8(uint8_t* out) { check(lgca_get_state_u8(dev(), out)); }
+    void set_state_u8(const uint8_t* in) { check(lgca_set_state_u8(dev(), in)); }
+    // the handle, with every step asked for so far enqueued
+    lgca_lattice* handle() { return dev(); }
 
     // --------------------------------------------------------------- getters
     Real nu_s() const { return m_nu_s; }
@@ -355,10 +361,24 @@ private:
             !m_mean_momentum_cpu)
             die("out of host memory");
     }
+    void flush()
+    {
+        if (m_pending) {
+            const uint64_t n = m_pending;
+            m_pending = 0;
+            check(lgca_step(m_h, n));
+        }
+    }
+    // The handle for a device call: steps still counted are enqueued first.
+    lgca_lattice* dev()
+    {
+        flush();
+        return m_h;
+    }
     void sync_types()
     {
         if (m_types_dirty) {
-            check(lgca_set_cell_types(m_h, m_cell_type_cpu));
+            check(lgca_set_cell_types(dev(), m_cell_type_cpu));
             m_types_dirty = false;
         }
     }
@@ -371,6 +391,8 @@ private:
     }
 
     lgca_lattice* m_h = nullptr;
+    static constexpr uint64_t kStepBatch = 64;
+    uint64_t m_pending = 0;  // collide_and_propagate steps not yet enqueued
     int m_device = 0;
     std::string m_test_case;
     const Real m_rho = 1.0, m_c = 1.0;

@@ -100,8 +100,9 @@ class Lattice:
                      u=f32(0), Ma_s=f32(Ma_s or 0), Re=f32(Re or 0))
         self._p = p
         self.m_bf_dir = p["bf_dir"]
-        self._h = C.c_void_p()
-        check(lib().lgca_create(C.byref(self._h), self.model, int(dim_x), int(dim_y), r,
+        self._pending = 0
+        self._hd = C.c_void_p()
+        check(lib().lgca_create(C.byref(self._hd), self.model, int(dim_x), int(dim_y), r,
                                 int(device)))
         self._attach(r)
         self.set_seed(seed)
@@ -111,8 +112,9 @@ class Lattice:
         """A lattice resumed from a file written by save_checkpoint: same
         state, cell types, seed, step and forcing settings."""
         self = cls.__new__(cls)
-        self._h = C.c_void_p()
-        check(lib().lgca_load_checkpoint(C.byref(self._h), os.fsencode(path), int(device)))
+        self._pending = 0
+        self._hd = C.c_void_p()
+        check(lib().lgca_load_checkpoint(C.byref(self._hd), os.fsencode(path), int(device)))
         info = self.info()
         self.model = int(info.model)
         self.NUM_DIR = NUM_DIR[self.model]
@@ -167,10 +169,24 @@ class Lattice:
         self.m_mean_momentum_cpu = np.zeros(2 * self.m_num_coarse_cells, dtype=np.float32)
 
     # ---------------------------------------------------------------- handle
+    def _flush(self) -> None:
+        """Enqueues the steps collide_and_propagate has counted."""
+        if self._pending:
+            n, self._pending = self._pending, 0
+            check(lib().lgca_step(self._hd, n), self._hd)
+
+    @property
+    def _h(self):
+        """The handle for a device call, pending steps enqueued first."""
+        self._flush()
+        return self._hd
+
     def close(self) -> None:
-        if self._h:
-            lib().lgca_destroy(self._h)
-            self._h = C.c_void_p()
+        hd = getattr(self, "_hd", None)
+        if hd:
+            lib().lgca_destroy(hd)
+            self._hd = C.c_void_p()
+            self._pending = 0
 
     def __del__(self):
         try:
@@ -189,7 +205,7 @@ class Lattice:
         return self._h
 
     def _check(self, code: int) -> None:
-        check(code, self._h)
+        check(code, self._hd)
 
     def info(self) -> capi.Info:
         inf = capi.Info()
@@ -366,7 +382,14 @@ class Lattice:
         self.synchronize()
 
     def collide_and_propagate(self, p: bool = False) -> None:
-        self.step(1)
+        """One step.  Steps are counted and enqueued together (one lgca_step
+        of up to 64) before anything else reaches the device, so the tiled
+        kernel advances several steps per pass; results are those of single
+        steps."""
+        self.copy_data_to_device()
+        self._pending += 1
+        if self._pending >= 64:
+            self._flush()
 
     def apply_body_force(self, forcing: int) -> int:
         """Count-mode force in direction m_bf_dir (src/omp_lattice.cpp:254-346)."""

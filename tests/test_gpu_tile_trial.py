@@ -77,3 +77,4 @@ def test_trial_on_a_lattice_shorter_than_its_candidates():
     lat.step(4)
     assert lat.info().tile_rows in {2, 4, 8, 12}
     lat.close()
+
