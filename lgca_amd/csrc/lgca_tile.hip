@@ -114,8 +114,6 @@ __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 
 }  // namespace
 
-typedef const __attribute__((address_space(4))) int32_t* const_i32_ptr;
-
 // K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
 template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
 
@@ -224,26 +222,46 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         if (whi > W) return hits(wlo, W) || hits(0, whi - W);
         return hits(wlo, whi);
     };
-    // Row ranges of non-fluid words are read through the scalar cache.
-    const const_i32_ptr rr = (const_i32_ptr)a.row_range;
-    auto row_has_solids = [&](uint32_t row) -> bool {
-        return wave_hits(rr[row], rr[g.dim_y + row]);
+    // Strip mode: which input rows of the walk hold solids in this wave's
+    // words, 64 iterations at a time.  Lane l tests the row of iteration
+    // i0 + l against the row's range of non-fluid words (one vector load per
+    // bound), and the ballot gives bit l.  A scalar load per row instead would
+    // be waited for before each row's branch, and at the start of a launch
+    // every wave waits at once.
+    auto solid_rows = [&](uint32_t i0) -> uint64_t {
+        bool hit = false;
+        const uint32_t i = i0 + lane;
+        if (i < n_iter) {
+            const uint32_t r = wrap((int)(y0 + i) - K);
+            hit = wave_hits(a.row_range[r], a.row_range[g.dim_y + r]);
+        }
+        return __builtin_amdgcn_ballot_w64(hit);
     };
     const bool force_cols = a.fdec && wave_hits((int)a.fw0, (int)(a.fw0 + a.fwn));
     const bool force_mine = a.fdec && c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
     const uint32_t fvoff = force_mine ? (c.w - a.fw0) * 4u : kOutside;  // strip mode
     // Decide once per tile whether the mask and force paths can be needed
     // (segment mode: whenever the lattice has masks / forcing; rows differ
-    // per segment).
+    // per segment).  srows holds the solid rows of iterations 0..63.
     bool tile_solids = false, tile_force = false;
+    uint64_t srows = 0;
     if (SEG) {
         tile_solids = a.mask != nullptr;
         tile_force = a.fdec != nullptr;
-    } else if (a.mask || force_cols) {
-        for (uint32_t i = 0; i < n_iter; ++i) {
-            const uint32_t r = wrap((int)(y0 + i) - K);
-            if (a.mask && row_has_solids(r)) tile_solids = true;
-            if (force_cols && r >= a.fy0 && r < a.fy1) tile_force = true;
+    } else {
+        if (a.mask) {
+            srows = solid_rows(0);
+            tile_solids = srows != 0;
+            for (uint32_t i0 = 64; !tile_solids && i0 < n_iter; i0 += 64) tile_solids = solid_rows(i0) != 0;
+        }
+        for (uint32_t i0 = 0; force_cols && !tile_force && i0 < n_iter; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            bool f = false;
+            if (i < n_iter) {
+                const uint32_t r = wrap((int)(y0 + i) - K);
+                f = r >= a.fy0 && r < a.fy1;
+            }
+            tile_force = __builtin_amdgcn_ballot_w64(f) != 0;
         }
     }
 
@@ -280,7 +298,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     uint32_t nxt[7];
     uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
     {
-        if (t_solids && (SEG || row_has_solids(row_i))) {
+        if (t_solids && (SEG || (srows & 1u))) {
             nns = load_mask(row_i, 0);
             nsl = load_mask(row_i, 1);
         }
@@ -308,7 +326,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         if (i + 1 < n_iter) {
             const uint32_t rn = row_next;
             nns = nsl = 0;
-            if (t_solids && (SEG || row_has_solids(rn))) {
+            // the row of iteration i + 1; a new group of 64 is tested on entry
+            if (t_solids && !SEG && ((i + 1u) & 63u) == 0) srows = solid_rows(i + 1u);
+            if (t_solids && (SEG || ((srows >> ((i + 1u) & 63u)) & 1u))) {
                 nns = load_mask(rn, 0);
                 nsl = load_mask(rn, 1);
             }
