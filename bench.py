@@ -264,12 +264,13 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--workload", default="karman", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--steps-per-pass", type=int, default=None,
-                    help="steps fused per HBM pass (1, 2, 4, 8; 0 = one-step quad kernel)")
+                    help="steps fused per HBM pass (1, 2, 4, 8; 0 = one-step quad kernel; "
+                         "-1 = automatic, the default)")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--ensemble-jobs", type=int, default=512, help="0 skips the ensemble leg")
     ap.add_argument("--ensemble-sample", type=int, default=32)
@@ -285,8 +286,8 @@ def main() -> int:
     from lgca_amd import hiprt
     w = WORKLOADS[args.workload]
     lat = build_lattice(w)
-    if args.steps_per_pass is not None:
-        lat.set_tuning(args.steps_per_pass, args.tile_rows)
+    if args.steps_per_pass is not None or args.tile_rows:
+        lat.set_tuning(-1 if args.steps_per_pass is None else args.steps_per_pass, args.tile_rows)
     # the step kernels run on this stream, so events recorded on it bracket them
     stream = hiprt.Stream()
     lat.set_stream(stream.ptr.value)
@@ -316,6 +317,7 @@ def main() -> int:
     if os.path.exists(tfile) and args.steps_per_pass is None and args.tile_rows == 0:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    info = lat.info()
     rec = {
         "metric": "site updates/s (GSUP/s)",
         "value": round(sups / 1e9, 3),
@@ -331,8 +333,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": w["label"], "cells": n, "directions": Q[w["model"]],
                    "parallelism": "single GPU",
-                   "steps_per_pass": 4 if args.steps_per_pass is None else args.steps_per_pass,
-                   "tile_rows": lat.info().tile_rows},
+                   "steps_per_pass": info.steps_per_pass, "tile_rows": info.tile_rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},

@@ -68,6 +68,10 @@ typedef struct lgca_info {
     uint64_t device_bytes;   /* everything the handle holds on the GPU        */
     uint64_t step;           /* absolute step counter (hash input)            */
     uint64_t seed;
+    int32_t steps_per_pass;  /* steps per launch of lgca_step: lgca_set_tuning's
+                                value, else the one the trial chose (2 or 4;
+                                -1 until it has)                                */
+    uint32_t reserved;
 } lgca_info;
 
 /* Creates a lattice of dim_x x dim_y cells on GPU `device`, all FLUID and
@@ -133,11 +137,13 @@ LGCA_API lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint32
 LGCA_API lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps);
 
 /* Launch shape of lgca_step: steps fused per pass over HBM (1, 2, 4 or 8; 0
- * selects the one-step kernel that keeps a 128-cell quad per thread) and rows
- * per tile (0 = automatic: a timed trial of a few heights on the first
- * launches, kept thereafter and restarted by this call, lgca_set_forcing, new
- * cell types or a new stream).  Results do not depend on it.  No counterpart in the
- * reference; used by the autotuner and benchmarks. */
+ * selects the one-step kernel that keeps a 128-cell quad per thread; -1, the
+ * default, is 4, or 2 where the trial finds 2-step passes faster on a lattice
+ * too small to fill the GPU) and rows per tile (0 = automatic: a timed trial
+ * of a few shapes on the first launches, kept thereafter and restarted by this
+ * call, lgca_set_forcing, new cell types or a new stream).  Results do not
+ * depend on it.  No counterpart in the reference; used by the autotuner and
+ * benchmarks. */
 LGCA_API lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows);
 
 /* Count-mode body force: replaces apply_body_force(int forcing)

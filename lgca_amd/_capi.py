@@ -29,7 +29,7 @@ class Info(C.Structure):
         ("coarse_dim_y", C.c_uint32), ("words_per_row", C.c_uint32), ("row_pitch", C.c_uint32),
         ("tile_rows", C.c_uint32), ("num_cells", C.c_uint64), ("num_coarse_cells", C.c_uint64),
         ("state_bytes", C.c_uint64), ("device_bytes", C.c_uint64), ("step", C.c_uint64),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("steps_per_pass", C.c_int32), ("reserved", C.c_uint32),
     ]
 
 

@@ -29,6 +29,14 @@ struct Controller {
     uint32_t calls;     // count-mode force call counter (hash stream 3)
     float v[2];         // mean velocity the last decision saw
 };
+
+// Steps per pass and rows per tile of one tiled launch.
+struct LaunchShape {
+    int k;
+    uint32_t th;
+};
+// steps_per_pass value that lets the trial choose it
+constexpr int kAutoSteps = -1;
 }  // namespace lgca
 
 struct lgca_lattice {
@@ -60,15 +68,17 @@ struct lgca_lattice {
     int force_dir = LGCA_FORCE_NONE;
     uint64_t force_thr = 0;
     uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
-    int tile_k = 4;                 // steps per pass of the tiled kernel (0: quad kernel)
+    int tile_k = kAutoSteps;        // steps per pass of the tiled kernel (0: quad kernel)
     uint32_t tile_th = 0;           // rows per tile, 0 = automatic
-    // Automatic rows per tile: a timed trial of a few heights on the first
-    // tile_k-step launches (two each), then the fastest (tile_rows_for_launch)
-    std::vector<uint32_t> trial_th; // candidate heights, the rule's own first
-    std::vector<hipEvent_t> trial_ev;  // start / end of every timed launch
-    uint32_t trial_issued = 0;      // timed launches queued so far
-    int trial_k = 0;
-    uint32_t tuned_th = 0;          // the chosen height, 0 while undecided
+    // Automatic launch shape: a timed trial of a few (steps per pass, rows per
+    // tile) pairs on the first launches, then the fastest (plan_launch)
+    std::vector<LaunchShape> trial; // candidates, the rule's own first
+    std::vector<hipEvent_t> trial_ev;  // start / end of every timed group of launches
+    std::vector<uint32_t> trial_steps; // steps launched inside each group
+    uint32_t trial_issued = 0;      // candidate launches queued so far
+    uint32_t trial_group = 1;       // launches per timed group
+    int trial_open = -1;            // group whose end event is still to come
+    LaunchShape tuned{0, 0};        // the chosen shape, th 0 while undecided
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     // forcing controller of the viewers (lgca_controller_*)
@@ -118,13 +128,13 @@ lgca_status hip_fail(lgca_lattice* h, hipError_t e, const char* what)
         if (e_ != hipSuccess) return hip_fail((h), e_, #expr);      \
     } while (0)
 
-// Starts the tile-height trial afresh (the workload or the stream changed).
+// Starts the launch-shape trial afresh (the workload or the stream changed).
 void reset_trial(lgca_lattice* h)
 {
-    h->trial_th.clear();
+    h->trial.clear();
     h->trial_issued = 0;
-    h->trial_k = 0;
-    h->tuned_th = 0;
+    h->trial_open = -1;
+    h->tuned = LaunchShape{0, 0};
 }
 
 template <typename T>
@@ -402,7 +412,7 @@ extern "C" lgca_status lgca_get_info(const lgca_lattice* h, lgca_info* info)
     info->coarse_dim_y = h->dim_y / (2 * h->r);
     info->words_per_row = h->g.W;
     info->row_pitch = h->g.P;
-    info->tile_rows = h->tile_th ? h->tile_th + (h->tile_th & 1u) : h->tuned_th;
+    info->tile_rows = h->tile_th ? h->tile_th + (h->tile_th & 1u) : h->tuned.th;
     info->num_cells = (uint64_t)h->dim_x * h->dim_y;
     info->num_coarse_cells = (uint64_t)info->coarse_dim_x * info->coarse_dim_y;
     info->state_bytes = (uint64_t)h->state_words * 4;
@@ -417,6 +427,8 @@ extern "C" lgca_status lgca_get_info(const lgca_lattice* h, lgca_info* info)
     info->device_bytes = dev;
     info->step = h->step;
     info->seed = h->seed;
+    if (h->tile_k != kAutoSteps) info->steps_per_pass = h->tile_k;
+    else info->steps_per_pass = h->tile_th ? 4 : (h->tuned.th ? h->tuned.k : -1);
     return LGCA_OK;
 }
 
@@ -687,70 +699,121 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
     return (uint32_t)th;
 }
 
-// Rows per tile for one k-step launch of the tiled kernel, and the trial slot
-// to time it in (-1: none).  With automatic tile rows, the first launches at
-// the configured steps per pass each run a candidate height (the rule's, then
-// 4, 8, 12, 16, 20, 24, 32 rows, two launches each) between two events on the
-// lattice's stream; once the last has finished (queried, never waited for)
-// the fastest height is kept, the rule's unless another beats it by more
-// than 2 %.  The rule is fixed on the 32768 x 8192 lattice, where its 20 rows
-// measured best, but other shapes gain up to a fifth from taller or shorter
-// tiles.  Results do not depend on the tiling.
-constexpr uint32_t kTrialReps = 2;
-static lgca_status tile_rows_for_launch(lgca_lattice* h, int k, uint32_t* th, int* slot)
+// The largest steps per launch, at most `want`, that the hash alignment
+// allows: K >= 4 needs step % 4 == 0, K == 2 an even step.
+static int steps_for_launch(int want, uint64_t step, uint64_t left)
 {
-    *slot = -1;
-    *th = auto_tile_rows(h, k);
-    if (h->tile_th || k != h->tile_k) return LGCA_OK;
-    if (h->tuned_th) {
-        *th = h->tuned_th;
+    if (want >= 8 && left >= 8 && (step & 3) == 0) return 8;
+    if (want >= 4 && left >= 4 && (step & 3) == 0) return 4;
+    if (want >= 2 && left >= 2 && (step & 1) == 0) return 2;
+    return 1;
+}
+
+// Shape of the next tiled launch (steps k, rows per tile th), and the trial
+// group whose start event goes before it and whose end event goes after it
+// (-1: none).  With automatic rows, the first launches run candidate shapes,
+// each twice in a group of consecutive launches between two events on the
+// lattice's stream: at the configured steps per pass (4 when automatic) the
+// rule's height, then 4, 8, 12, 16, 20, 24, 32 rows; and, with automatic
+// steps on a lattice the rule gives 2-row tiles, 2-step passes of 2 and 4
+// rows, whose shorter walks suit lattices too small to fill the GPU (the
+// 2048 x 512 pipe runs 10 % faster so).  Groups are single launches, or four
+// when 2-step passes compete: then the gaps between launches must count as
+// they do in a long run, and timing each launch alone penalised the shorter
+// 2-step launches.  Once the last group has
+// finished (queried, never waited for) the fastest per step is kept, the
+// rule's unless another beats it by more than 2 %.  The rule is fixed on the
+// 32768 x 8192 lattice, where its 20 rows measured best, but other shapes gain
+// up to a fifth from taller or shorter tiles.  Results do not depend on the
+// shape.
+constexpr uint32_t kTrialReps = 2;
+static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t* th, int* ev_start,
+                               int* ev_end)
+{
+    *ev_start = *ev_end = -1;
+    const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
+    if (h->tile_th) {
+        *k = steps_for_launch(base_k, h->step, left);
+        *th = auto_tile_rows(h, *k);
         return LGCA_OK;
     }
-    if (h->trial_th.empty() || h->trial_k != k) {
-        h->trial_th.assign(1, *th);
-        for (uint32_t c : {4u, 8u, 12u, 16u, 20u, 24u, 32u})
-            if (c <= h->dim_y && std::find(h->trial_th.begin(), h->trial_th.end(), c) == h->trial_th.end())
-                h->trial_th.push_back(c);
-        h->trial_k = k;
+    if (!h->tuned.th && h->trial.empty()) {
+        h->trial.assign(1, LaunchShape{base_k, auto_tile_rows(h, base_k)});
+        auto add = [&](int ck, uint32_t cth) {
+            if (cth > h->dim_y) return;
+            for (const LaunchShape& s : h->trial)
+                if (s.k == ck && s.th == cth) return;
+            h->trial.push_back(LaunchShape{ck, cth});
+        };
+        for (uint32_t c : {4u, 8u, 12u, 16u, 20u, 24u, 32u}) add(base_k, c);
+        h->trial_group = 1;
+        if (h->tile_k == kAutoSteps && h->trial[0].th == 2) {
+            add(2, 2);
+            add(2, 4);
+            h->trial_group = 4;
+        }
         h->trial_issued = 0;
-        const size_t need = 2 * kTrialReps * h->trial_th.size();
-        while (h->trial_ev.size() < need) {
+        h->trial_open = -1;
+        const size_t groups = kTrialReps * h->trial.size();
+        while (h->trial_ev.size() < 2 * groups) {
             hipEvent_t ev;
             HIPCHK(h, hipEventCreate(&ev));
             h->trial_ev.push_back(ev);
         }
+        h->trial_steps.assign(groups, 0);
     }
-    const uint32_t n = kTrialReps * (uint32_t)h->trial_th.size();
-    if (h->trial_issued < n) {
-        *slot = (int)h->trial_issued;
-        *th = h->trial_th[h->trial_issued / kTrialReps];
-        ++h->trial_issued;
+    const uint32_t groups = kTrialReps * (uint32_t)h->trial.size();
+    const uint32_t n = h->trial_group * groups;  // candidate launches in all
+    if (!h->tuned.th && h->trial_issued == n && h->trial_open < 0) {
+        const hipError_t q = hipEventQuery(h->trial_ev[2 * (groups - 1) + 1]);
+        if (q != hipErrorNotReady) {
+            HIPCHK(h, q);
+            std::vector<float> best(h->trial.size(), 1e30f);
+            for (uint32_t e = 0; e < groups; ++e) {
+                float ms = 0.0f;
+                HIPCHK(h, hipEventElapsedTime(&ms, h->trial_ev[2 * e], h->trial_ev[2 * e + 1]));
+                const size_t c = e / kTrialReps;
+                best[c] = std::min(best[c], ms / (float)std::max<uint32_t>(h->trial_steps[e], 1));
+            }
+            size_t pick = 0;
+            for (size_t c = 1; c < best.size(); ++c)
+                if (best[c] < best[pick]) pick = c;
+            if (best[0] <= 1.02f * best[pick]) pick = 0;
+            h->tuned = h->trial[pick];
+        }
+    }
+    if (h->tuned.th) {
+        *k = steps_for_launch(h->tuned.k, h->step, left);
+        *th = *k == h->tuned.k ? h->tuned.th : auto_tile_rows(h, *k);
         return LGCA_OK;
     }
-    const hipError_t q = hipEventQuery(h->trial_ev[2 * (n - 1) + 1]);
-    if (q == hipErrorNotReady) return LGCA_OK;  // the rule's height meanwhile
-    HIPCHK(h, q);
-    std::vector<float> best(h->trial_th.size(), 1e30f);
-    for (uint32_t i = 0; i < n; ++i) {
-        float ms = 0.0f;
-        HIPCHK(h, hipEventElapsedTime(&ms, h->trial_ev[2 * i], h->trial_ev[2 * i + 1]));
-        best[i / kTrialReps] = std::min(best[i / kTrialReps], ms);
+    if (h->trial_issued < n) {
+        const uint32_t e = h->trial_issued / h->trial_group, pos = h->trial_issued % h->trial_group;
+        const LaunchShape& c = h->trial[e / kTrialReps];
+        *k = steps_for_launch(c.k, h->step, left);
+        if (*k == c.k) {
+            // a launch the alignment forces to another k inside a group is
+            // timed with it, and its steps are counted
+            *th = c.th;
+            if (pos == 0) *ev_start = (int)e;
+            if (pos + 1 == h->trial_group) *ev_end = (int)e;
+            ++h->trial_issued;
+            return LGCA_OK;
+        }
+    } else {
+        *k = steps_for_launch(base_k, h->step, left);  // the rule's shape meanwhile
     }
-    size_t pick = 0;
-    for (size_t c = 1; c < best.size(); ++c)
-        if (best[c] < best[pick]) pick = c;
-    if (best[0] <= 1.02f * best[pick]) pick = 0;
-    h->tuned_th = h->trial_th[pick];
-    *th = h->tuned_th;
+    *th = auto_tile_rows(h, *k);
     return LGCA_OK;
 }
 
 extern "C" lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows)
 {
     if (!h) return LGCA_ERR_INVALID_ARGUMENT;
-    if (steps_per_pass != 0 && steps_per_pass != 1 && steps_per_pass != 2 && steps_per_pass != 4 &&
-        steps_per_pass != 8)
-        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_tuning: steps_per_pass must be 0, 1, 2, 4 or 8");
+    if (steps_per_pass != kAutoSteps && steps_per_pass != 0 && steps_per_pass != 1 && steps_per_pass != 2 &&
+        steps_per_pass != 4 && steps_per_pass != 8)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT,
+                    "lgca_set_tuning: steps_per_pass must be -1, 0, 1, 2, 4 or 8");
     if (tile_rows > 65534)
         return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_tuning: tile_rows must be at most 65534");
     std::lock_guard<std::mutex> lk(h->mu);
@@ -778,13 +841,12 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps)
     const size_t per_step = (size_t)(fw1 - fw0) * (h->fy1 - h->fy0);
     uint64_t left = nsteps;
     while (left > 0) {
-        // largest block the hash alignment allows: K >= 4 needs step % 4 == 0,
-        // K == 2 an even step
-        int k = 1;
-        if (h->tile_k >= 2 && left >= 2 && (h->step & 1) == 0) k = 2;
-        if (h->tile_k >= 4 && left >= 4 && (h->step & 3) == 0) k = 4;
-        if (h->tile_k >= 8 && left >= 8 && (h->step & 3) == 0) k = 8;
-        if (h->tile_k == 0) k = 1;
+        int k = 1, ev_start = -1, ev_end = -1;
+        uint32_t th = 0;
+        if (h->tile_k != 0) {
+            lgca_status st = plan_launch(h, left, &k, &th, &ev_start, &ev_end);
+            if (st) return st;
+        }
         const uint32_t* fdec = nullptr;
         if (forcing) {
             if (h->dec_count == 0 || h->step < h->dec_step0 ||
@@ -821,9 +883,7 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps)
             a.key0 = (uint32_t)h->seed;
             a.key1 = (uint32_t)(h->seed >> 32);
             a.t0 = (uint32_t)h->step;
-            int slot = -1;
-            lgca_status st = tile_rows_for_launch(h, k, &a.th, &slot);
-            if (st) return st;
+            a.th = th;
             a.halo = tile_halo(h->g, k);
             a.fdec = fdec;
             a.fdec_stride = per_step;
@@ -832,9 +892,17 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps)
             a.fw0 = fw0;
             a.fwn = fw1 - fw0;
             a.force_dir = h->force_dir;
-            if (slot >= 0) HIPCHK(h, hipEventRecord(h->trial_ev[2 * slot], h->stream));
+            if (ev_start >= 0) {
+                HIPCHK(h, hipEventRecord(h->trial_ev[2 * ev_start], h->stream));
+                h->trial_open = ev_start;
+                h->trial_steps[ev_start] = 0;
+            }
             HIPCHK(h, launch_tile(h->kind, k, a, h->stream));
-            if (slot >= 0) HIPCHK(h, hipEventRecord(h->trial_ev[2 * slot + 1], h->stream));
+            if (h->trial_open >= 0) h->trial_steps[h->trial_open] += (uint32_t)k;
+            if (ev_end >= 0) {
+                HIPCHK(h, hipEventRecord(h->trial_ev[2 * ev_end + 1], h->stream));
+                h->trial_open = -1;
+            }
         }
         h->cur ^= 1;
         h->step += (uint64_t)k;

@@ -238,7 +238,8 @@ class Lattice:
 
     def set_tuning(self, steps_per_pass: int, tile_rows: int = 0) -> None:
         """Launch shape of step(): steps fused per HBM pass (1, 2, 4, 8; 0 = the
-        one-step quad kernel) and rows per tile (0 = automatic)."""
+        one-step quad kernel; -1 = automatic, the default) and rows per tile
+        (0 = automatic)."""
         self._check(lib().lgca_set_tuning(self._h, int(steps_per_pass), int(tile_rows)))
 
     def step(self, nsteps: int = 1) -> None:

@@ -1,6 +1,6 @@
-"""The timed trial of tile heights on large lattices (lgca_set_tuning with
-automatic rows): it settles on one of its candidates, never changes results,
-and restarts when the workload changes."""
+"""The timed trial of launch shapes (lgca_set_tuning with automatic rows, and
+automatic steps per pass by default): it settles on one of its candidates,
+never changes results, and restarts when the workload changes."""
 import sys
 
 import numpy as np
@@ -41,20 +41,23 @@ def test_trial_picks_a_candidate_and_keeps_results():
     ref, _, _ = make_lattice(dx, dy, seed)
     ref.set_tuning(4, 20)
     assert lat.info().tile_rows == 0
-    # 8 candidates x 2 launches x 4 steps = 64 steps of trial, in uneven calls
-    for n in (5, 3, 17, 9, 30, 1, 22, 13):
+    assert lat.info().steps_per_pass == -1
+    # 8 candidates x 2 groups x 4 launches x 4 steps, and 2 x 2 x 4 x 2 steps
+    # at K = 2 (this lattice's rule gives 2-row tiles): 288 steps of trial, in
+    # uneven calls
+    for n in (5, 3, 17, 9, 30, 1, 22, 13, 7, 11, 101, 41, 161):
         lat.step(n)
         ref.step(n)
     lat.synchronize()
     lat.step(4)  # decides now that the trial's launches have finished
     ref.step(4)
-    rows = lat.info().tile_rows
-    say(f"[trial] chosen tile rows {rows}")
-    assert rows in CANDIDATES
+    rows, k = lat.info().tile_rows, lat.info().steps_per_pass
+    say(f"[trial] chosen tile rows {rows}, steps per pass {k}")
+    assert (k == 4 and rows in CANDIDATES) or (k == 2 and rows in {2, 4})
     np.testing.assert_array_equal(lat.get_state_u8(), ref.get_state_u8())
-    assert lat.steps == ref.steps == 104
+    assert lat.steps == ref.steps == 425
     # and both agree with the oracle
-    want = orc.run(2, dx, dy, types, state, seed, 0, 104,
+    want = orc.run(2, dx, dy, types, state, seed, 0, 425,
                    dict(dir="x", p=0.02, x0=0, x1=200), nthreads=8)
     np.testing.assert_array_equal(lat.get_state_u8(), want)
     # a new forcing restarts the trial; an explicit height is reported as set
@@ -72,9 +75,52 @@ def test_trial_on_a_lattice_shorter_than_its_candidates():
     lat = Lattice(2, dim_x=1024, dim_y=14, coarse_graining_radius=1, seed=3)
     lat.apply_bc_periodic()
     lat.init_random(0.3)
-    lat.step(80)
+    lat.step(200)
     lat.synchronize()
     lat.step(4)
     assert lat.info().tile_rows in {2, 4, 8, 12}
     lat.close()
 
+
+
+def test_automatic_steps_per_pass_on_a_small_lattice():
+    """The 2048 x 512 pipe (forced, walls): the default shape trial also times
+    2-step passes; whatever it keeps, results equal a fixed shape's and the
+    oracle's, and an explicit steps_per_pass is kept."""
+    from lgca_amd import Lattice
+    dx, dy, seed = 2048, 512, 7
+
+    def pipe():
+        lat = Lattice(2, dim_x=dx, dim_y=dy, coarse_graining_radius=1, seed=seed)
+        lat.apply_bc_pipe()
+        lat.init_random(0.3)
+        lat.set_forcing("x", 1e-3)
+        return lat
+
+    lat, ref = pipe(), pipe()
+    ref.set_tuning(4, 8)
+    types, state = lat.m_cell_type_cpu.copy(), lat.get_state_u8()
+    np.testing.assert_array_equal(state, ref.get_state_u8())
+    say(f"[trial] {dx}x{dy} FHP-II pipe, automatic steps per pass")
+    for n in (3, 150, 1, 29, 164):
+        lat.step(n)
+        ref.step(n)
+    lat.synchronize()
+    lat.step(6)
+    ref.step(6)
+    info = lat.info()
+    say(f"[trial] chosen steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
+    assert (info.steps_per_pass, info.tile_rows) in {(4, r) for r in CANDIDATES} | {(2, 2), (2, 4)}
+    got = lat.get_state_u8()
+    np.testing.assert_array_equal(got, ref.get_state_u8())
+    want = orc.run(2, dx, dy, types, state, seed, 0, 353, dict(dir="x", p=1e-3, x0=0, x1=dx), nthreads=8)
+    np.testing.assert_array_equal(got, want)
+    # an explicit steps per pass keeps the trial to heights at that K
+    lat.set_tuning(4, 0)
+    assert lat.info().steps_per_pass == 4 and lat.info().tile_rows == 0
+    lat.step(300)
+    lat.synchronize()
+    lat.step(4)
+    assert lat.info().steps_per_pass == 4 and lat.info().tile_rows in CANDIDATES
+    lat.close()
+    ref.close()
