@@ -240,14 +240,39 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     const bool force_cols = a.fdec && wave_hits((int)a.fw0, (int)(a.fw0 + a.fwn));
     const bool force_mine = a.fdec && c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
     const uint32_t fvoff = force_mine ? (c.w - a.fw0) * 4u : kOutside;  // strip mode
-    // Decide once per tile whether the mask and force paths can be needed
-    // (segment mode: whenever the lattice has masks / forcing; rows differ
-    // per segment).  srows holds the solid rows of iterations 0..63.
+    // K <= 2: the walk's first input row is loaded before the tests below, so
+    // that their loads wait out one memory latency together with these (short
+    // walks on small lattices feel it).  At K = 4 the registers this holds
+    // across the tests push the walks into spilling, so the walk loads it.
+    constexpr bool kEarlyRow = K <= 2;
+    const uint32_t row_first = wrap((int)y0 - K);
+    uint32_t first[7];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) first[d] = kEarlyRow ? load_plane(row_first, d) : 0u;
+
+    // Decide once per tile whether the mask and force paths can be needed.
+    // srows holds the solid rows of iterations 0..63 (strip mode).
     bool tile_solids = false, tile_force = false;
     uint64_t srows = 0;
     if (SEG) {
-        tile_solids = a.mask != nullptr;
+        // Segments walk consecutive row tiles, so rows [wy0 - K, wy0 + S th
+        // + K) cover every row the live ones read; a wave with no solid row
+        // there takes the lean walk.  Forcing: whenever the lattice has it
+        // (rows differ per segment).
         tile_force = a.fdec != nullptr;
+        if (a.mask) {
+            const uint32_t S = 64u / g.W;
+            const uint32_t wy0 = tile * S * a.th, span = S * a.th + 2u * (uint32_t)K;
+            for (uint32_t i0 = 0; !tile_solids && i0 < span; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                bool hit = false;
+                if (i < span) {
+                    const uint32_t r = wrap((int)(wy0 + i) - K);
+                    hit = a.row_range[r] < a.row_range[g.dim_y + r];
+                }
+                tile_solids = __builtin_amdgcn_ballot_w64(hit) != 0;
+            }
+        }
     } else {
         if (a.mask) {
             srows = solid_rows(0);
@@ -290,7 +315,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
 
     // rows: row_i is the input row of the current iteration; lrow[s] the row
     // level s works on (row_i - s on the torus)
-    uint32_t row_i = wrap((int)y0 - K);
+    uint32_t row_i = row_first;
     uint32_t lrow[K + 1];
 #pragma unroll
     for (int s = 1; s <= K; ++s) lrow[s] = wrap((int)row_i - s);
@@ -303,7 +328,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
             nsl = load_mask(row_i, 1);
         }
 #pragma unroll
-        for (int d = 0; d < Q; ++d) nxt[d] = load_plane(row_i, d);
+        for (int d = 0; d < Q; ++d) nxt[d] = kEarlyRow ? first[d] : load_plane(row_i, d);
     }
 
     // One step of the row walk.  STEADY iterations (i >= 2K) have every level
