@@ -176,6 +176,46 @@ def test_results_do_not_depend_on_tuning_or_mix(tuning):
     check_job("job 5 alone", alone[0], want[5])
 
 
+def strip_jobs(rng):
+    """Jobs wider than 64 words, which the batch runs in x-strips: three
+    strips, two with a ragged last word, given types with solids in a few
+    rows only, forcing in one column band."""
+    from lgca_amd import Job
+    sparse = np.zeros(3968 * 96, dtype=np.uint8)
+    g = sparse.reshape(96, 3968)
+    for r in (1, 40, 41, 94):
+        g[r, 700:760] = 1
+    g[60, -5:] = 2
+    g[rng.random((96, 3968)) < 0.0005] = 1
+    return [
+        Job(2, 4160, 300, 29, seed=51, fill=0.3, geometry="pipe", force_dir="x", force_p=0.01,
+            coarse_radius=16),
+        Job(1, 2100, 140, 23, seed=52, fill=0.25, geometry="cylinder", cyl_x=2000, cyl_y=70,
+            cyl_diameter=40.0, coarse_radius=5),
+        Job(0, 3968, 96, 31, seed=53, fill=0.2, geometry="types", cell_types=sparse, coarse_radius=8),
+        Job(3, 4096, 130, 17, seed=54, fill=0.3, geometry="cylinder", cyl_x=100, cyl_y=120,
+            cyl_diameter=30.0, force_dir="y", force_p=0.05, force_region=(3000, 3100, 0, 130),
+            coarse_radius=13),
+    ]
+
+
+def test_strip_mode_jobs_match_oracle():
+    """Jobs in x-strips (rows over 64 words, the serving sizes) under the
+    autotuner, under tall tiles (over 64 rows: the walk's solid-row mask is
+    refreshed mid-tile) and under short 2-step tiles: every job equals the
+    oracle."""
+    rng = np.random.default_rng(50)
+    jobs = strip_jobs(rng)
+    want = [oracle_job(j) for j in jobs]
+    for tuning in (None, (4, 16, 0), (2, 100000, 0), (8, 0, 0)):
+        say(f"[batch] strip-mode jobs, tuning {tuning}")
+        got, stats = run_batch(jobs, tuning=tuning)
+        assert stats["jobs_done"] == len(jobs)
+        for i, job in enumerate(jobs):
+            check_job(f"strip job {i} {job.model}/{job.dim_x}x{job.dim_y}/{job.geometry} "
+                      f"tuning {tuning}", got[i], want[i])
+
+
 def test_budget_admission_and_release():
     """With a budget that holds two jobs, the rest wait in order and are
     admitted as finished jobs are released; a job larger than the budget is

@@ -47,6 +47,12 @@ def make_types(rng, dx, dy, kind):
         g[0, :] = g[-1, :] = 1
         ys, xs = np.mgrid[0:dy, 0:dx]
         g[(xs - dx // 3) ** 2 + (ys - dy // 2) ** 2 < (dy // 4) ** 2] = 1
+    elif kind == "sparse":
+        # solids in a few rows only, two of them next to the y wrap: most
+        # waves of a tall lattice see none, the edge ones see them across it
+        for r in (1, 70, 71, 130 % dy, dy - 2):
+            g[r, dx // 5: dx // 5 + 9] = 1
+        g[dy // 2, -3:] = 2
     return t
 
 
@@ -98,10 +104,16 @@ STEP_CASES = [
     # tall narrow rows: many row tiles per wave (segment mode), idle lanes
     (2, 96, 300, "mixed", dict(dir="x", p=0.1, y0=50, y1=250)),
     (0, 700, 50, "cylinder", None),
+    # solids in a few rows: waves with and without solid rows side by side
+    # (segment mode with 8 and 2 rows per wave; strip mode, where 100-row
+    # tiles test rows past the first 64 of the walk)
+    (2, 256, 200, "sparse", None),
+    (1, 1024, 130, "sparse", dict(dir="y", p=0.2, y0=100, y1=130)),
+    (2, 4096, 200, "sparse", dict(dir="x", p=0.1, y0=60, y1=80)),
 ]
 
 
-TUNINGS = [(0, 0), (1, 0), (2, 0), (4, 0), (8, 0), (4, 5), (8, 3)]
+TUNINGS = [(0, 0), (1, 0), (2, 0), (4, 0), (8, 0), (4, 5), (8, 3), (4, 100), (2, 2)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"k{t[0]}th{t[1]}")
