@@ -808,6 +808,35 @@ uint32_t tile_rows_for(uint64_t wave_rows_total, uint32_t waves)
     return (uint32_t)th;
 }
 
+// Whether a work item reading rows [a, b) of a job (on the torus) can take
+// the general walk: rows with walls, the disc, given cell types or forcing.
+// It only orders the items of a class (those first, so that waves that run
+// longer start first and the rest fill in behind them); results never depend
+// on it.
+bool rows_heavy(const Job* j, int64_t a, int64_t b)
+{
+    const int64_t H = (int64_t)j->d.dim_y;
+    if (b - a >= H) {
+        a = 0;
+        b = H;
+    }
+    auto hits = [&](int64_t lo, int64_t hi) {  // [lo, hi) inside [0, H)
+        auto span = [&](int64_t s0, int64_t s1) { return s0 < hi && s1 > lo; };
+        if (j->p.forced && span(j->p.fy0, j->p.fy1)) return true;
+        if (j->d.geometry == LGCA_GEOM_PERIODIC) return false;
+        if (j->d.geometry == LGCA_GEOM_TYPES) return true;
+        if (span(0, 1) || span(H - 1, H)) return true;
+        if (j->d.geometry == LGCA_GEOM_CYLINDER && !j->halfw.empty()) {
+            const int64_t nh = (int64_t)j->halfw.size(), cy = j->d.cyl_y;
+            if (span(cy - nh + 1, cy + nh)) return true;
+        }
+        return false;
+    };
+    const int64_t a0 = ((a % H) + H) % H, len = b - a;
+    if (a0 + len <= H) return hits(a0, a0 + len);
+    return hits(a0, H) || hits(0, a0 + len - H);
+}
+
 // Builds this round's work items if the mix changed.
 void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
 {
@@ -818,7 +847,7 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     if (b->sig == b->last_sig) return;
     b->last_sig = b->sig;
     // classes in a fixed order: kind, k, ragged (segment and strip jobs share)
-    std::map<uint64_t, std::vector<WorkItem>> cls;
+    std::map<uint64_t, std::vector<WorkItem>> cls, light;  // light: appended after
     for (size_t i = 0; i < n; ++i) {
         const Job* j = b->active[i];
         const Geometry& g = j->p.g;
@@ -828,17 +857,28 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         const uint32_t jth = std::min<uint32_t>(th, g.dim_y + (g.dim_y & 1u));
         const uint32_t tiles = (g.dim_y + jth - 1) / jth;
         const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) | (g.nb != 32 ? 1u : 0u);
-        std::vector<WorkItem>& v = cls[key];
+        std::vector<WorkItem>& vh = cls[key];
+        std::vector<WorkItem>& vl = light[key];
         if (seg) {
             const uint32_t S = 64 / g.W, groups = (tiles + S - 1) / S;
-            for (uint32_t t = 0; t < groups; ++t)
+            for (uint32_t t = 0; t < groups; ++t) {
+                const int64_t y0 = (int64_t)t * S * jth;
+                std::vector<WorkItem>& v = rows_heavy(j, y0 - k, y0 + (int64_t)S * jth + k) ? vh : vl;
                 v.push_back(WorkItem{(uint32_t)j->slot, 0u, t, jth | (1u << 31), groups});
+            }
         } else {
             const uint32_t strips = tile_strips(g, k);
-            for (uint32_t t = 0; t < tiles; ++t)
+            for (uint32_t t = 0; t < tiles; ++t) {
+                const int64_t y0 = (int64_t)t * jth, rows = std::min<int64_t>(jth, g.dim_y - y0);
+                std::vector<WorkItem>& v = rows_heavy(j, y0 - k, y0 + rows + k) ? vh : vl;
                 for (uint32_t s = 0; s < strips; ++s)
                     v.push_back(WorkItem{(uint32_t)j->slot, s, t, jth | (halo << 16), tiles * strips});
+            }
         }
+    }
+    for (auto& kv : light) {
+        std::vector<WorkItem>& v = cls[kv.first];
+        v.insert(v.end(), kv.second.begin(), kv.second.end());
     }
     size_t total = 0;
     for (auto& kv : cls) total += kv.second.size();
