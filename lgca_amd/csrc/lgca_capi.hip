@@ -687,11 +687,12 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
     // rows x waves per row: strips per row, or 1 / (rows per wave) in segment mode
     const uint64_t wave_rows = tile_segments(h->g, k) ? (uint64_t)h->dim_y / (64 / h->g.W)
                                                       : (uint64_t)h->dim_y * tile_strips(h->g, k);
-    // about 1.75 rounds of resident waves (four per SIMD on 1024 SIMDs); on the
-    // 32768 x 8192 lattice this gives 20-row tiles, the fastest measured.  Small
-    // lattices get short tiles: a wave's walk of th + 2K rows is then their
-    // critical path (2-row tiles run the 2048 x 512 pipe 1.6x faster than 8)
-    uint64_t th = wave_rows / 7168;
+    // about 2.1 rounds of resident waves (four per SIMD on 1024 SIMDs); on the
+    // 32768 x 8192 lattice this gives 16-row tiles, the fastest measured with
+    // the XCD-contiguous block order.  Small lattices get short tiles: a
+    // wave's walk of th + 2K rows is then their critical path (2-row tiles run
+    // the 2048 x 512 pipe 1.6x faster than 8)
+    uint64_t th = wave_rows / 8704;
     if (th < 2) th = 2;
     if (th > 256) th = 256;
     if (th > h->dim_y) th = h->dim_y;
