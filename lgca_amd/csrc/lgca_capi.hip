@@ -721,12 +721,11 @@ static int steps_for_launch(int want, uint64_t step, uint64_t left)
 // 2048 x 512 pipe runs 10 % faster so).  Groups are single launches, or four
 // when 2-step passes compete: then the gaps between launches must count as
 // they do in a long run, and timing each launch alone penalised the shorter
-// 2-step launches.  Once the last group has
-// finished (queried, never waited for) the fastest per step is kept, the
-// rule's unless another beats it by more than 2 %.  The rule is fixed on the
-// 32768 x 8192 lattice, where its 20 rows measured best, but other shapes gain
-// up to a fifth from taller or shorter tiles.  Results do not depend on the
-// shape.
+// 2-step launches.  Once the last group has finished (queried, never waited
+// for) the fastest per step is kept, the rule's unless another beats it by
+// more than 2 %.  The rule is fixed on the 32768 x 8192 lattice, where its 16
+// rows measured best, but other shapes gain up to a fifth from taller or
+// shorter tiles.  Results do not depend on the shape.
 constexpr uint32_t kTrialReps = 2;
 static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t* th, int* ev_start,
                                int* ev_end)
