@@ -560,10 +560,13 @@ void tile_kernel(TileArgs a, TileOrder o)
     // Workgroups go to the 8 XCDs round-robin.  Each XCD takes a contiguous
     // run of the order instead, so the tiles it holds are vertical neighbours
     // whose halo rows meet in its L2: 32768 x 8192 Karman 6830 -> 6990
-    // GSUP/s at the best tile height of each mapping.
-    const uint32_t xcd = blockIdx.x & 7u, q = gridDim.x >> 3, r = gridDim.x & 7u;
-    const uint32_t L = xcd * q + min(xcd, r) + (blockIdx.x >> 3);
+    // GSUP/s at the best tile height of each mapping.  The heavy and the light
+    // blocks are mapped apart, so that every XCD gets its share of both.
     const uint32_t H = o.nheavy * o.ntiles;
+    const bool in_heavy = blockIdx.x < H;
+    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - H, n = in_heavy ? H : gridDim.x - H;
+    const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
+    const uint32_t L = (in_heavy ? 0u : H) + xcd * q + min(xcd, r) + (b >> 3);
     uint32_t bx, tile;
     if (L < H) {
         tile = L / o.nheavy;
