@@ -368,6 +368,7 @@ struct ClassRange {
     int kind, k;
     bool ragged;
     uint32_t offset, count;
+    uint32_t heavy;  // leading items that can take the general walk
 };
 
 struct Candidate {
@@ -876,8 +877,10 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
             }
         }
     }
+    std::map<uint64_t, uint32_t> nheavy;
     for (auto& kv : light) {
         std::vector<WorkItem>& v = cls[kv.first];
+        nheavy[kv.first] = (uint32_t)v.size();
         v.insert(v.end(), kv.second.begin(), kv.second.end());
     }
     size_t total = 0;
@@ -897,6 +900,7 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         c.ragged = (kv.first & 1u) != 0;
         c.offset = (uint32_t)all.size();
         c.count = (uint32_t)kv.second.size();
+        c.heavy = nheavy[kv.first];
         b->classes.push_back(c);
         all.insert(all.end(), kv.second.begin(), kv.second.end());
     }
@@ -1071,7 +1075,7 @@ void run_round(lgca_batch* b)
             side_used[j] = true;
         }
         hipchk(launch_tile_batch(c.kind, c.k, c.ragged, b->d_jobs, b->d_state,
-                                 b->d_items[b->items_buf] + c.offset, c.count, s),
+                                 b->d_items[b->items_buf] + c.offset, c.count, c.heavy, s),
                "tile_batch_kernel");
         ++nl;
     }

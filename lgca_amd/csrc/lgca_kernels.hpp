@@ -91,8 +91,9 @@ struct WorkItem {
 // advances its job K steps reading its BatchState.  The wave that finishes a
 // job's last item of the round moves the job K steps on and swaps its buffers,
 // for the next round's launches to read.
+// n_heavy: the class's leading items that can take the general walk.
 hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
-                             const WorkItem* items, uint32_t n_items, hipStream_t s);
+                             const WorkItem* items, uint32_t n_items, uint32_t n_heavy, hipStream_t s);
 
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]

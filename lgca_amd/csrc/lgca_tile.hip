@@ -26,6 +26,7 @@
 #include "lgca_dev.hpp"
 #include "lgca_kernels.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace lgca {
@@ -591,11 +592,18 @@ void tile_kernel(TileArgs a, TileOrder o)
 template <int KIND, int K, bool RAGGED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
-                       const WorkItem* __restrict__ items, uint32_t n_items)
+                       const WorkItem* __restrict__ items, uint32_t n_items, uint32_t heavy_blocks)
 {
+    // Blocks go to the 8 XCDs round-robin; the leading (heavy) blocks and the
+    // rest each map to contiguous runs per XCD, as in tile_kernel.
+    const bool in_heavy = blockIdx.x < heavy_blocks;
+    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - heavy_blocks;
+    const uint32_t n = in_heavy ? heavy_blocks : gridDim.x - heavy_blocks;
+    const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
+    const uint32_t blk = (in_heavy ? 0u : heavy_blocks) + xcd * q + min(xcd, r) + (b >> 3);
     // wave-uniform by construction; readfirstlane lets the compiler keep the
     // job record in scalar registers
-    const uint32_t wi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const uint32_t wi = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
     if (wi >= n_items) return;
     const WorkItem it = items[wi];
     const BatchState bs = state[it.job];
@@ -623,16 +631,17 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
 }
 
 hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
-                             const WorkItem* items, uint32_t n_items, hipStream_t s)
+                             const WorkItem* items, uint32_t n_items, uint32_t n_heavy, hipStream_t s)
 {
     if (!n_items) return hipSuccess;
-    const dim3 grid((n_items + 3) / 4);
+    const uint32_t nblk = (n_items + 3) / 4, hblk = std::min<uint32_t>((n_heavy + 3) / 4, nblk);
+    const dim3 grid(nblk);
 #define LGCA_BATCH_CASE(KD, KK)                                                                   \
     if (kind == KD && k == KK) {                                                                \
         if (ragged) hipLaunchKernelGGL((tile_batch_kernel<KD, KK, true>), grid, dim3(256), 0, s,  \
-                                       jobs, state, items, n_items);                            \
+                                       jobs, state, items, n_items, hblk);                      \
         else hipLaunchKernelGGL((tile_batch_kernel<KD, KK, false>), grid, dim3(256), 0, s, jobs,  \
-                                state, items, n_items);                                         \
+                                state, items, n_items, hblk);                                   \
         return hipGetLastError();                                                               \
     }
     LGCA_BATCH_CASE(kHPP, 1) LGCA_BATCH_CASE(kHPP, 2) LGCA_BATCH_CASE(kHPP, 4) LGCA_BATCH_CASE(kHPP, 8)
