@@ -560,9 +560,11 @@ void tile_kernel(TileArgs a, TileOrder o)
     }
     // Workgroups go to the 8 XCDs round-robin.  Each XCD takes a contiguous
     // run of the order instead, so the tiles it holds are vertical neighbours
-    // whose halo rows meet in its L2: 32768 x 8192 Karman 6830 -> 6990
-    // GSUP/s at the best tile height of each mapping.  The heavy and the light
-    // blocks are mapped apart, so that every XCD gets its share of both.
+    // whose halo rows meet in its L2 (32768 x 8192 Karman: 6830 -> 6990
+    // GSUP/s at the best tile height of each mapping).  The heavy and the
+    // light blocks are mapped apart, so that every XCD gets its share of both
+    // (7086 -> 7452: mapped as one range, the forced blocks all landed on the
+    // first XCDs, which then finished last).
     const uint32_t H = o.nheavy * o.ntiles;
     const bool in_heavy = blockIdx.x < H;
     const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - H, n = in_heavy ? H : gridDim.x - H;
