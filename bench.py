@@ -6,8 +6,15 @@
 Default workload: FHP-III Karman street, 32768 x 8192 cells (268,435,456),
 NO_SLIP cylinder of diameter 512 at (4096, 4099), walls on rows 0 and 8191,
 fill 0.25 per direction, per-cell x force p = 2e-3 on x < 64, hash seed 11.
-A step is one collide-and-propagate time step of the whole lattice; K steps
-are timed after W warm-up steps, with the state resident in HBM throughout.
+A step is one collide-and-propagate time step of the whole lattice.  The
+launch shape is tuned first (lgca_tune: timed passes that leave the state and
+the step unchanged), then K steps are timed after W warm-up steps, with the
+state resident in HBM throughout.
+
+The other two single-lattice workloads are timed the same way, at their full
+step counts, and ride on the line under "single_lattice":
+  HPP 256x256 periodic, fill 0.2, seed 1, 1000 steps;
+  FHP-II pipe 2048x512, walls, fill 0.3, x force p=1e-3, seed 7, 20000 steps.
 
 Two more measurements ride on the same JSON line:
   "ensemble": 512 FHP-II jobs (long side 128..2048 step 64, aspect 1..4,
@@ -70,11 +77,36 @@ def build_lattice(w):
     return lat
 
 
+def host_cpu():
+    """(threads to use, CPUs this process may run on, CPUs of the machine, CPU
+    model).  The thread count is the process's CPU share: OMP_NUM_THREADS when
+    set (the GPU machines set it to their share), else the CPUs in its
+    affinity mask."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    threads = affinity
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, affinity, os.cpu_count() or 1, model
+
+
 def cpu_baseline(w, lat, step0: int, budget_s: float):
     """The CPU oracle (a restatement of the reference's OpenMP step) on the same
     lattice and state, for as many steps as fit in about `budget_s` seconds."""
     from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    threads, affinity, ncpu, cpu_model = host_cpu()
     state = lat.get_state_u8()
     types = lat.m_cell_type_cpu.copy()
     model = {"HPP": 0, "FHP_I": 1, "FHP_II": 2, "FHP_III": 3}[w["model"]]
@@ -96,9 +128,68 @@ def cpu_baseline(w, lat, step0: int, budget_s: float):
         t1 += time.perf_counter() - t
         steps += more
     return {"value": round(n * steps / t1 / 1e9, 4), "unit": "GSUP/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "nproc": affinity, "machine_cpus": ncpu, "cpu_model": cpu_model,
             "sample": f"{steps} step(s) of the same {w['dim_x']}x{w['dim_y']} workload from "
-                      f"step {step0}, {t1:.1f} s, oracle/cpu_ref.c with {threads} OpenMP threads"}
+                      f"step {step0}, {t1:.1f} s, oracle/cpu_ref.c (byte per cell, pull, "
+                      f"OpenMP over rows) with {threads} OpenMP threads, the process's CPU "
+                      f"share ({affinity} CPUs in its affinity mask, {ncpu} in the machine)"}
+
+
+def time_steps(w, steps: int, warmup: int, tuning=None, tune: bool = True):
+    """One lattice of workload w: set up, tune the launch shape, warm up, then
+    time `steps` steps with HIP events on the stream its kernels run on."""
+    from lgca_amd import hiprt
+    lat = build_lattice(w)
+    if tuning is not None:
+        lat.set_tuning(*tuning)
+    # the step kernels run on this stream, so events recorded on it bracket them
+    stream = hiprt.Stream()
+    lat.set_stream(stream.ptr.value)
+    if tune:
+        lat.tune()
+    n = w["dim_x"] * w["dim_y"]
+    lat.step(warmup)
+    stream.synchronize()
+    n0 = lat.get_n_particles()
+    start, end = hiprt.Event(), hiprt.Event()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    lat.step(steps)
+    end.record(stream)
+    end.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_ms(end)
+    n1 = lat.get_n_particles()
+    info = lat.info()
+    lat.close()
+    start.close()
+    end.close()
+    stream.close()
+    sups = n * steps / (ms / 1e3)
+    achieved = n * 2 * Q[w["model"]] / 8 * steps / (ms / 1e3) / 1e9
+    return dict(ms=ms, wall=wall, sups=sups, achieved=achieved, conserved=n0 == n1,
+                steps_per_pass=int(info.steps_per_pass), tile_rows=int(info.tile_rows))
+
+
+def other_configs():
+    """The HPP 256x256 and FHP-II 2048x512 workloads at their full step counts."""
+    out = {}
+    for key, name, steps in (("hpp_256x256_periodic", "hpp", 1000),
+                             ("fhp2_pipe_2048x512", "pipe", 20000)):
+        w = WORKLOADS[name]
+        r = time_steps(w, steps, warmup=8)
+        out[key] = {
+            "workload": w["label"], "steps": steps, "ms_per_step": round(r["ms"] / steps, 6),
+            "value": round(r["sups"] / 1e9, 3), "unit": "GSUP/s",
+            "steps_per_pass": r["steps_per_pass"], "tile_rows": r["tile_rows"],
+            "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(r["achieved"] / HBM_PEAK_GBS, 4)},
+            "particles_conserved": r["conserved"],
+        }
+        print(f"[bench] {key}: {out[key]['value']} GSUP/s", file=sys.stderr, flush=True)
+    return out
 
 
 # ------------------------------------------------------------ ensembles
@@ -268,6 +359,8 @@ def main() -> int:
     ap.add_argument("--workload", default="karman", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the HPP 256x256 and FHP-II 2048x512 single-lattice rates")
     ap.add_argument("--steps-per-pass", type=int, default=None,
                     help="steps fused per HBM pass (1, 2, 4, 8; 0 = one-step quad kernel; "
                          "-1 = automatic, the default)")
@@ -283,44 +376,25 @@ def main() -> int:
               "(the multi-GPU y-strip decomposition is a design, not a build)", file=sys.stderr)
         return 2
 
-    from lgca_amd import hiprt
     w = WORKLOADS[args.workload]
-    lat = build_lattice(w)
+    tuning = None
     if args.steps_per_pass is not None or args.tile_rows:
-        lat.set_tuning(-1 if args.steps_per_pass is None else args.steps_per_pass, args.tile_rows)
-    # the step kernels run on this stream, so events recorded on it bracket them
-    stream = hiprt.Stream()
-    lat.set_stream(stream.ptr.value)
+        tuning = (-1 if args.steps_per_pass is None else args.steps_per_pass, args.tile_rows)
+    r = time_steps(w, args.steps, args.warmup, tuning)
     n = w["dim_x"] * w["dim_y"]
-
-    lat.step(args.warmup)
-    stream.synchronize()
-    n0 = lat.get_n_particles()
-    start, end = hiprt.Event(), hiprt.Event()
-    stream.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    lat.step(args.steps)
-    end.record(stream)
-    end.synchronize()
-    wall = time.perf_counter() - t0
-    ms = start.elapsed_ms(end)
-    n1 = lat.get_n_particles()
-
-    sups = n * args.steps / (ms / 1e3)
-    bytes_per_step = n * 2 * Q[w["model"]] / 8
-    achieved = bytes_per_step * args.steps / (ms / 1e3) / 1e9
+    ms = r["ms"]
     # HBM bytes per launch of the dominant kernel, measured with rocprofv3
     # FETCH_SIZE / WRITE_SIZE for the default launch shape (see the file)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"step_kernel_{args.workload}_traffic.json")
-    if os.path.exists(tfile) and args.steps_per_pass is None and args.tile_rows == 0:
+    if os.path.exists(tfile) and tuning is None:
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    info = lat.info()
+            tr = json.load(f)
+        if (tr.get("steps_per_pass"), tr.get("tile_rows")) == (r["steps_per_pass"], r["tile_rows"]):
+            traffic = tr.get("hbm_bytes_per_launch")
     rec = {
         "metric": "site updates/s (GSUP/s)",
-        "value": round(sups / 1e9, 3),
+        "value": round(r["sups"] / 1e9, 3),
         "unit": "GSUP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -333,17 +407,15 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": w["label"], "cells": n, "directions": Q[w["model"]],
                    "parallelism": "single GPU",
-                   "steps_per_pass": info.steps_per_pass, "tile_rows": info.tile_rows},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                   "steps_per_pass": r["steps_per_pass"], "tile_rows": r["tile_rows"]},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
-        "particles_conserved": n0 == n1,
-        "wall_s": round(wall, 3),
+        "particles_conserved": r["conserved"],
+        "wall_s": round(r["wall"], 3),
     }
-    lat.close()
-    start.close()
-    end.close()
-    stream.close()
+    if not args.no_other_configs:
+        rec["single_lattice"] = other_configs()
     if args.ensemble_jobs > 0:
         print("[bench] ensemble", file=sys.stderr, flush=True)
         rec["ensemble"] = ensemble_bench(args.ensemble_jobs, args.ensemble_sample)
@@ -357,7 +429,7 @@ def main() -> int:
         rec["cpu_baseline"] = cpu_baseline(w, lat, 0, args.cpu_seconds)
         lat.close()
     print(json.dumps(rec), flush=True)
-    return 0 if n0 == n1 else 1
+    return 0 if r["conserved"] else 1
 
 
 if __name__ == "__main__":

@@ -145,6 +145,14 @@ LGCA_API lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps);
  * depend on it.  No counterpart in the reference; used by the autotuner and
  * benchmarks. */
 LGCA_API lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows);
+/* Runs that trial now and waits for it: every candidate shape is timed on
+ * passes that read the live state and write the other state buffer, and the
+ * fastest per step is kept (lgca_get_info reports it).  The state and the
+ * step do not change.  Does nothing when lgca_set_tuning fixed the rows per
+ * tile or selected the one-step kernel.  No counterpart in the reference;
+ * benchmarks call it before their warm-up, so that timed steps run the kept
+ * shape from the first. */
+LGCA_API lgca_status lgca_tune(lgca_lattice* h);
 
 /* Count-mode body force: replaces apply_body_force(int forcing)
  * (src/omp_lattice.cpp:254-346).  Candidate cells come from the hash (stream

@@ -18,6 +18,9 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblgca_hip.so")
 
 SOURCES = ["lgca_kernels.hip", "lgca_tile.hip", "lgca_capi.hip", "lgca_batch.hip"]
+# lgca_tile_inst.hip is compiled once per (model kind, steps per pass) pair,
+# which lets the template-heavy tiled kernels compile in parallel
+TILE_INST = [(kind, k) for k in (8, 4, 2, 1) for kind in (0, 1, 2)]
 ARCH = os.environ.get("LGCA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -39,19 +42,29 @@ def _hipcc() -> str:
     return os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(job) -> str:
+    src, defines, name = job
+    obj = os.path.join(BUILD, name + ".o")
+    cmd = [_hipcc(), *HIPCC_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"hipcc failed on {src} {' '.join(defines)}:\n{r.stdout}\n{r.stderr}")
     return obj
+
+
+def _jobs():
+    jobs = [(f"lgca_tile_inst.hip", [f"-DLGCA_INST_KIND={kind}", f"-DLGCA_INST_K={k}"],
+             f"lgca_tile_m{kind}_k{k}") for kind, k in TILE_INST]
+    jobs += [(src, [], os.path.splitext(src)[0]) for src in SOURCES]
+    return jobs
 
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    jobs = _jobs()
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -700,13 +700,14 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
     return (uint32_t)th;
 }
 
-// The largest steps per launch, at most `want`, that the hash alignment
-// allows: K >= 4 needs step % 4 == 0, K == 2 an even step.
-static int steps_for_launch(int want, uint64_t step, uint64_t left)
+// The largest steps per launch, at most `want` and `left`.  A pass may start
+// at any step: the tiled kernels are compiled for each offset of the first
+// step within its four-step chirality group.
+static int steps_for_launch(int want, uint64_t left)
 {
-    if (want >= 8 && left >= 8 && (step & 3) == 0) return 8;
-    if (want >= 4 && left >= 4 && (step & 3) == 0) return 4;
-    if (want >= 2 && left >= 2 && (step & 1) == 0) return 2;
+    if (want >= 8 && left >= 8) return 8;
+    if (want >= 4 && left >= 4) return 4;
+    if (want >= 2 && left >= 2) return 2;
     return 1;
 }
 
@@ -727,31 +728,39 @@ static int steps_for_launch(int want, uint64_t step, uint64_t left)
 // rows measured best, but other shapes gain up to a fifth from taller or
 // shorter tiles.  Results do not depend on the shape.
 constexpr uint32_t kTrialReps = 2;
+
+// The trial's candidate shapes, the rule's own first.
+static std::vector<LaunchShape> trial_candidates(const lgca_lattice* h)
+{
+    const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
+    std::vector<LaunchShape> c(1, LaunchShape{base_k, auto_tile_rows(h, base_k)});
+    auto add = [&](int ck, uint32_t cth) {
+        if (cth > h->dim_y) return;
+        for (const LaunchShape& s : c)
+            if (s.k == ck && s.th == cth) return;
+        c.push_back(LaunchShape{ck, cth});
+    };
+    for (uint32_t r : {4u, 8u, 12u, 16u, 20u, 24u, 32u}) add(base_k, r);
+    if (h->tile_k == kAutoSteps && c[0].th == 2) {
+        add(2, 2);
+        add(2, 4);
+    }
+    return c;
+}
+
 static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t* th, int* ev_start,
                                int* ev_end)
 {
     *ev_start = *ev_end = -1;
     const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
     if (h->tile_th) {
-        *k = steps_for_launch(base_k, h->step, left);
+        *k = steps_for_launch(base_k, left);
         *th = auto_tile_rows(h, *k);
         return LGCA_OK;
     }
     if (!h->tuned.th && h->trial.empty()) {
-        h->trial.assign(1, LaunchShape{base_k, auto_tile_rows(h, base_k)});
-        auto add = [&](int ck, uint32_t cth) {
-            if (cth > h->dim_y) return;
-            for (const LaunchShape& s : h->trial)
-                if (s.k == ck && s.th == cth) return;
-            h->trial.push_back(LaunchShape{ck, cth});
-        };
-        for (uint32_t c : {4u, 8u, 12u, 16u, 20u, 24u, 32u}) add(base_k, c);
-        h->trial_group = 1;
-        if (h->tile_k == kAutoSteps && h->trial[0].th == 2) {
-            add(2, 2);
-            add(2, 4);
-            h->trial_group = 4;
-        }
+        h->trial = trial_candidates(h);
+        h->trial_group = h->trial.back().k == 2 && base_k != 2 ? 4 : 1;
         h->trial_issued = 0;
         h->trial_open = -1;
         const size_t groups = kTrialReps * h->trial.size();
@@ -783,17 +792,17 @@ static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t*
         }
     }
     if (h->tuned.th) {
-        *k = steps_for_launch(h->tuned.k, h->step, left);
+        *k = steps_for_launch(h->tuned.k, left);
         *th = *k == h->tuned.k ? h->tuned.th : auto_tile_rows(h, *k);
         return LGCA_OK;
     }
     if (h->trial_issued < n) {
         const uint32_t e = h->trial_issued / h->trial_group, pos = h->trial_issued % h->trial_group;
         const LaunchShape& c = h->trial[e / kTrialReps];
-        *k = steps_for_launch(c.k, h->step, left);
+        *k = steps_for_launch(c.k, left);
         if (*k == c.k) {
-            // a launch the alignment forces to another k inside a group is
-            // timed with it, and its steps are counted
+            // a launch that the steps left force to another k inside a
+            // group is timed with it, and its steps are counted
             *th = c.th;
             if (pos == 0) *ev_start = (int)e;
             if (pos + 1 == h->trial_group) *ev_end = (int)e;
@@ -801,7 +810,7 @@ static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t*
             return LGCA_OK;
         }
     } else {
-        *k = steps_for_launch(base_k, h->step, left);  // the rule's shape meanwhile
+        *k = steps_for_launch(base_k, left);  // the rule's shape meanwhile
     }
     *th = auto_tile_rows(h, *k);
     return LGCA_OK;
@@ -823,7 +832,106 @@ extern "C" lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint
     return LGCA_OK;
 }
 
+// Arguments of one tiled pass from the live state into the other buffer, at
+// the current step, with rows per tile th and the force decisions of that
+// step (or null).
+static TileArgs tile_args(const lgca_lattice* h, uint32_t th, const uint32_t* fdec)
+{
+    const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
+    TileArgs a{};
+    a.g = h->g;
+    a.src = h->state[h->cur];
+    a.dst = h->state[h->cur ^ 1];
+    a.mask = h->mask;
+    a.row_range = h->row_range;
+    a.key0 = (uint32_t)h->seed;
+    a.key1 = (uint32_t)(h->seed >> 32);
+    a.t0 = (uint32_t)h->step;
+    a.th = th;
+    a.fdec = fdec;
+    a.fdec_stride = (uint64_t)(fw1 - fw0) * (h->fy1 - h->fy0);
+    a.fy0 = h->fy0;
+    a.fy1 = h->fy1;
+    a.fw0 = fw0;
+    a.fwn = fw1 - fw0;
+    a.force_dir = h->force_dir;
+    return a;
+}
+
+// The halo of a pass of k steps (a TileArgs from tile_args with it set).
+static TileArgs with_k(TileArgs a, int k)
+{
+    a.halo = tile_halo(a.g, k);
+    return a;
+}
+
 static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps);
+
+// The launch-shape trial of plan_launch, run at once and waited for: each
+// candidate shape is timed over groups of consecutive passes covering eight
+// steps (three interleaved rounds, the best of each kept), and the fastest
+// per step is kept, the rule's unless another beats it by more than 2 %.
+// Every candidate pass reads the live state and writes the other buffer,
+// which the next step overwrites anyway, so neither the state nor the step
+// changes.
+constexpr uint32_t kTuneReps = 3;
+extern "C" lgca_status lgca_tune(lgca_lattice* h)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    if (h->tile_k == 0 || h->tile_th) return LGCA_OK;  // no automatic choice to make
+    const std::vector<LaunchShape> cand = trial_candidates(h);
+    int kmax = 1;
+    for (const LaunchShape& c : cand) kmax = std::max(kmax, c.k);
+    const uint32_t* fdec = nullptr;
+    if (h->force_dir != LGCA_FORCE_NONE) {
+        if (h->dec_count == 0 || h->step < h->dec_step0 || h->step + kmax > h->dec_step0 + h->dec_count) {
+            lgca_status st = prepare_decisions(h, 64, (uint32_t)kmax);
+            if (st) return st;
+        }
+        fdec = h->dec + (size_t)(h->step - h->dec_step0) * tile_args(h, 2, nullptr).fdec_stride;
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    lgca_status st = LGCA_OK;
+    auto fail_hip = [&](hipError_t e, const char* what) {
+        if (st == LGCA_OK && e != hipSuccess) st = hip_fail(h, e, what);
+        return e != hipSuccess;
+    };
+    if (!fail_hip(hipEventCreate(&ev[0]), "hipEventCreate") && !fail_hip(hipEventCreate(&ev[1]), "hipEventCreate")) {
+        std::vector<float> best(cand.size(), 1e30f);
+        for (uint32_t rep = 0; rep < kTuneReps && st == LGCA_OK; ++rep) {
+            for (size_t i = 0; i < cand.size() && st == LGCA_OK; ++i) {
+                const TileArgs a = with_k(tile_args(h, cand[i].th, fdec), cand[i].k);
+                const int n = 8 / cand[i].k;
+                // the first pass of a shape loads its code: untimed
+                if (rep == 0 && fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream), "launch_tile")) break;
+                if (fail_hip(hipEventRecord(ev[0], h->stream), "hipEventRecord")) break;
+                for (int l = 0; l < n && st == LGCA_OK; ++l)
+                    fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream), "launch_tile");
+                if (st || fail_hip(hipEventRecord(ev[1], h->stream), "hipEventRecord") ||
+                    fail_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize"))
+                    break;
+                float ms = 0.0f;
+                if (fail_hip(hipEventElapsedTime(&ms, ev[0], ev[1]), "hipEventElapsedTime")) break;
+                best[i] = std::min(best[i], ms / 8.0f);
+            }
+        }
+        if (st == LGCA_OK) {
+            size_t pick = 0;
+            for (size_t i = 1; i < best.size(); ++i)
+                if (best[i] < best[pick]) pick = i;
+            if (best[0] <= 1.02f * best[pick]) pick = 0;
+            h->trial.clear();
+            h->trial_issued = 0;
+            h->trial_open = -1;
+            h->tuned = cand[pick];
+        }
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return st;
+}
 
 extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
 {
@@ -874,30 +982,12 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps)
             a.step = (uint32_t)h->step;
             HIPCHK(h, launch_step(h->kind, a, h->stream));
         } else {
-            TileArgs a{};
-            a.g = h->g;
-            a.src = h->state[h->cur];
-            a.dst = h->state[h->cur ^ 1];
-            a.mask = h->mask;
-            a.row_range = h->row_range;
-            a.key0 = (uint32_t)h->seed;
-            a.key1 = (uint32_t)(h->seed >> 32);
-            a.t0 = (uint32_t)h->step;
-            a.th = th;
-            a.halo = tile_halo(h->g, k);
-            a.fdec = fdec;
-            a.fdec_stride = per_step;
-            a.fy0 = h->fy0;
-            a.fy1 = h->fy1;
-            a.fw0 = fw0;
-            a.fwn = fw1 - fw0;
-            a.force_dir = h->force_dir;
             if (ev_start >= 0) {
                 HIPCHK(h, hipEventRecord(h->trial_ev[2 * ev_start], h->stream));
                 h->trial_open = ev_start;
                 h->trial_steps[ev_start] = 0;
             }
-            HIPCHK(h, launch_tile(h->kind, k, a, h->stream));
+            HIPCHK(h, launch_tile(h->kind, k, with_k(tile_args(h, th, fdec), k), h->stream));
             if (h->trial_open >= 0) h->trial_steps[h->trial_open] += (uint32_t)k;
             if (ev_end >= 0) {
                 HIPCHK(h, hipEventRecord(h->trial_ev[2 * ev_end + 1], h->stream));

@@ -49,8 +49,7 @@ struct TileArgs {
     const uint32_t* mask;
     const int32_t* row_range;
     uint32_t key0, key1;
-    uint32_t t0;                // first step of the block (mod 2^32); K >= 4 needs t0 % 4 == 0,
-                                // K == 2 needs t0 even
+    uint32_t t0;                // first step of the pass (mod 2^32), any value
     uint32_t th;                // rows per tile
     uint32_t halo;              // lanes per wave edge that are computed but not stored
     const uint32_t* fdec;       // decisions of step t0, or null
@@ -64,6 +63,17 @@ uint32_t tile_strips(const Geometry& g, int k);  // x-strips per row tile in str
 // row tile), used when it beats strips (S >= 2, or a row needing two strips).
 bool tile_segments(const Geometry& g, int k);
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s);
+
+// Strip-mode block order of tile_kernel: the heavy block columns (those with
+// a forced strip, at most kMaxHeavy, ascending) of every row tile are
+// dispatched before the others (nheavy 0: plain order).
+constexpr uint32_t kMaxHeavy = 8;
+struct TileOrder {
+    uint32_t nbx;               // block columns (four strips each)
+    uint32_t ntiles;            // row tiles
+    uint32_t nheavy;            // heavy block columns
+    uint32_t heavy[kMaxHeavy];  // their indices, ascending
+};
 
 // Batched ensembles.  BatchJob is a job's static descriptor (its TileArgs
 // without src/dst/t0/th/halo, which come from BatchState and WorkItem);
@@ -94,6 +104,16 @@ struct WorkItem {
 // n_heavy: the class's leading items that can take the general walk.
 hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
                              const WorkItem* items, uint32_t n_items, uint32_t n_heavy, hipStream_t s);
+
+// Launchers of one (model, K) pair, instantiated by lgca_tile_inst.hip; grid
+// and shapes come checked from launch_tile / launch_tile_batch.
+template <int KIND, int K>
+hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblocks, bool ragged,
+                          bool seg, hipStream_t s);
+template <int KIND, int K>
+hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
+                                uint32_t n_items, uint32_t nblocks, uint32_t heavy_blocks, bool ragged,
+                                hipStream_t s);
 
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]

@@ -1,0 +1,681 @@
+// lgca_tile.hpp -- temporally blocked collide-and-propagate on gfx950.
+//
+// One pass over HBM advances the lattice K steps.  A wave owns 64 consecutive
+// 32-cell words of a row (one word per lane) and marches down a tile of TH
+// rows; for every input row it loads (7 plane words per lane) it runs the K
+// time levels as a register pipeline: level s turns rows j-1, j, j+1 of level
+// s-1 into row j of level s, with j lagging the input row by s.  Only level 0
+// is read from HBM and only level K is written back, so the state crosses HBM
+// once per K steps.
+//
+// x-halo: a shifted plane needs one bit of the neighbouring word, which the
+// neighbouring lane holds (DPP wave_shr / wave_shl).  Lanes 0 and 63 have no
+// outer neighbour, so the cells next to the wave edges go wrong, and the wrong
+// zone grows by one cell per level.  Read along the lanes, the cells form the
+// row in order (a ragged last word contributes only its nb valid cells), so
+// after K levels the first and last K cells of the wave are suspect.  The
+// host picks a halo of h lanes per side holding at least K cells (h = 1 for
+// full words, 2 when an edge lane may hold a short last word, ceil(K / nb)
+// when the row is a single word); waves advance by 64 - 2h words and store
+// lanes h..63-h.
+// y-halo: the tile reads K extra rows above and below and discards them.
+//
+// Chirality: philox({w, y, t >> 2, 0})[t & 3] serves a word for the four
+// steps of a group.  A pass starting at step t0 is compiled for O = t0 & 3:
+// the first level of each group in the pass makes the call for its row, and
+// the group's later levels take their word from a delay line, so a pass makes
+// one call per word and row per group it touches, whatever its first step.
+//
+// The kernels are templates; lgca_tile_inst.hip instantiates one (model, K)
+// pair per translation unit, and lgca_tile.hip dispatches to them.
+#pragma once
+
+#include "lgca_dev.hpp"
+#include "lgca_kernels.hpp"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace lgca {
+
+namespace {
+
+__device__ __forceinline__ uint32_t from_left(uint32_t v)   // lane i gets lane i-1
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v)  // lane i gets lane i+1
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);
+}
+
+struct LaneCtx {
+    uint32_t w;          // word index of this lane (torus, 0..W-1)
+    uint32_t wb;         // 4 * w, the lane's byte offset within a plane row
+    uint32_t first_shl;  // 32 - nb on the lane holding word 0, else 0
+    bool last;           // lane holds word W-1
+    uint32_t nb;
+    uint32_t vmask;      // valid bits of this lane's word
+    uint32_t lsrc, rsrc; // segment mode: byte address of the lanes holding words w-1, w+1
+};
+
+// Memory goes through buffer resources: a wave-uniform base in scalar
+// registers, a 32-bit per-lane offset and a scalar offset, so no load or
+// store needs vector address arithmetic, and the hardware drops accesses past
+// `bytes`.  (Word 3 selects raw 32-bit access, as for every gfx9 buffer.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+// `soff` is wave-uniform at every call; readfirstlane says so to the
+// compiler, which otherwise may reuse a vector copy of the value and wrap the
+// access in a loop over its distinct values.
+__device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, __builtin_amdgcn_readfirstlane(soff), 0);
+}
+__device__ __forceinline__ void bstore(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, __builtin_amdgcn_readfirstlane(soff), 0);
+}
+
+// Neighbouring words: the adjacent lane (DPP) when a wave holds consecutive
+// words of one row, or the lane holding word w -+ 1 of the same row in
+// segment mode, where the row wraps inside its segment (ds_bpermute).
+template <bool SEG>
+__device__ __forceinline__ uint32_t left_word(uint32_t v, const LaneCtx& c)
+{
+    return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.lsrc, (int)v) : from_left(v);
+}
+template <bool SEG>
+__device__ __forceinline__ uint32_t right_word(uint32_t v, const LaneCtx& c)
+{
+    return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.rsrc, (int)v) : from_right(v);
+}
+
+// Output `sel` (wave-uniform) of a Philox call, by selects on the values
+// (a chain over an array would become a dynamically indexed stack array).
+__device__ __forceinline__ uint32_t pick4(const U4& r, uint32_t sel)
+{
+    const uint32_t lo = (sel & 1u) ? r.y : r.x;
+    const uint32_t hi = (sel & 1u) ? r.w : r.z;
+    return (sel & 2u) ? hi : lo;
+}
+
+// Chirality words of a pass whose first step t0 has t0 & 3 == O.  Level s
+// (step t0 + s - 1) uses word (O + s - 1) & 3 of its group's call.  The call
+// is made by the group's first level in the pass: level s itself when its
+// word is 0 or s == 1, else level max(1, s - word), which worked on the same
+// row that many iterations earlier.
+__host__ __device__ constexpr int chir_word(int O, int s) { return (O + s - 1) & 3; }
+__host__ __device__ constexpr bool chir_caller(int O, int s) { return s == 1 || chir_word(O, s) == 0; }
+__host__ __device__ constexpr int chir_src(int O, int s)
+{
+    return chir_caller(O, s) ? s : (s - chir_word(O, s) >= 1 ? s - chir_word(O, s) : 1);
+}
+__host__ __device__ constexpr int chir_delay(int O, int s) { return s - chir_src(O, s); }
+
+// out cell x = in cell x-1, cyclically over dim_x
+template <bool RAGGED, bool SEG>
+__device__ __forceinline__ uint32_t shift_m1(uint32_t v, const LaneCtx& c)
+{
+    uint32_t L = left_word<SEG>(v, c);
+    if (RAGGED) L <<= c.first_shl;
+    return __builtin_amdgcn_alignbit(v, L, 31);
+}
+// out cell x = in cell x+1, cyclically over dim_x
+template <bool RAGGED, bool SEG>
+__device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
+{
+    const uint32_t R = right_word<SEG>(v, c);
+    const uint32_t o = __builtin_amdgcn_alignbit(R, v, 1);
+    if (!RAGGED) return o;
+    return c.last ? ((v >> 1) | ((R & 1u) << (c.nb - 1u))) : o;
+}
+
+}  // namespace
+
+// K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
+template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
+
+// One wave's work.  Strip mode (SEG false): x-strip `strip` of row tile
+// `tile`.  Segment mode (SEG true, rows of W <= 64 words): the wave's lanes
+// form S = 64 / W segments of W lanes, each holding a whole row; segment s
+// walks row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when
+// K == 1, whose one level picks its word at run time).
+template <int KIND, int K, int O, bool RAGGED, bool SEG>
+__device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile)
+{
+    constexpr int Q = num_dir_of_kind(KIND);
+    const Geometry& g = a.g;
+    const uint32_t lane = threadIdx.x & 63u;
+    // tiles start on even rows (a.th is even), so the parity of the row a
+    // level works on follows from the iteration index: see `odd_row` below
+    uint32_t y0, rows;  // per lane in segment mode
+    int wabs;
+    bool writer;
+    LaneCtx c;
+    if (SEG) {
+        const uint32_t S = 64u / g.W;
+        const uint32_t ntiles = (g.dim_y + a.th - 1) / a.th;
+        if (tile * S >= ntiles) return;  // whole wave
+        const uint32_t seg = lane / g.W;  // S on the 64 - S * W idle lanes
+        const uint32_t base = seg * g.W;
+        const uint32_t mytile = tile * S + seg;
+        const bool live = seg < S && mytile < ntiles;
+        y0 = live ? mytile * a.th : 0u;
+        rows = live ? min(a.th, g.dim_y - y0) : 0u;
+        wabs = (int)(lane - base);  // < W, also on idle lanes
+        writer = live;
+        // (idle lanes may name a lane past 63; they never feed a live lane)
+        c.lsrc = 4u * ((wabs == 0 ? base + g.W - 1u : lane - 1u) & 63u);
+        c.rsrc = 4u * (((uint32_t)wabs + 1u == g.W ? base : lane + 1u) & 63u);
+    } else {
+        const uint32_t stride = 64u - 2u * a.halo;
+        const uint32_t nstrips = (g.W + stride - 1) / stride;
+        if (strip >= nstrips) return;  // whole wave
+        y0 = tile * a.th;
+        rows = min(a.th, g.dim_y - y0);
+        wabs = (int)(strip * stride) - (int)a.halo + (int)lane;
+        writer = lane >= a.halo && lane < 64u - a.halo && wabs < (int)g.W;
+        c.lsrc = c.rsrc = 0;
+    }
+    c.w = (uint32_t)((wabs + (int)g.W) % (int)g.W);
+    c.wb = 4u * c.w;
+    c.nb = g.nb;
+    c.first_shl = (c.w == 0) ? 32u - g.nb : 0u;
+    c.last = c.w + 1 == g.W;
+    c.vmask = (c.last && g.nb < 32) ? ((1u << g.nb) - 1u) : 0xFFFFFFFFu;
+    uint32_t edge = 0;  // cells of this lane on the east / west columns
+    if (c.w == 0) edge |= 1u;
+    if (c.last) edge |= 1u << (g.nb - 1u);
+
+    const uint32_t* src = a.src;
+    uint32_t* dst = a.dst;
+    const uint32_t n_iter = (SEG ? a.th : rows) + 2 * K;
+
+    // Plane d (or mask plane b) of row r at this lane's word.  Strip mode: the
+    // row is wave-uniform, so each access builds a descriptor over that row.
+    // Segment mode: rows differ per lane, so one descriptor spans the buffer
+    // (W <= 64 keeps it under 4 GiB) and the row goes into the lane offset.
+    const uint32_t plane_b = g.P * 4u, row_b = (uint32_t)Q * plane_b;
+    auto load_plane = [&](uint32_t r, int d) -> uint32_t {
+        if (SEG) return bload(make_rsrc(src, g.dim_y * row_b), r * row_b + c.wb, (uint32_t)d * plane_b);
+        return bload(make_rsrc(src + (size_t)r * Q * g.P, row_b), c.wb, (uint32_t)d * plane_b);
+    };
+    // Stores and force-decision loads are issued by every lane: a lane that
+    // must not access memory gets the offset kOutside, past any descriptor's
+    // size, and the hardware drops its store or returns 0 for its load.  An
+    // exec-masked branch around them would make the count of memory
+    // operations issued after the next row's loads path-dependent, and the
+    // waits for those loads would then wait for the stores as well.
+    constexpr uint32_t kOutside = 0x80000000u;
+    const uint32_t fplane_b = (a.fy1 - a.fy0) * a.fwn * 4u;  // one step's force decisions
+    auto store_plane = [&](uint32_t r, int d, uint32_t v, bool wr) {
+        if (SEG) bstore(v, make_rsrc(dst, g.dim_y * row_b), wr ? r * row_b + c.wb : kOutside, (uint32_t)d * plane_b);
+        else bstore(v, make_rsrc(dst + (size_t)r * Q * g.P, row_b), wr ? c.wb : kOutside, (uint32_t)d * plane_b);
+    };
+    auto load_mask = [&](uint32_t r, int b) -> uint32_t {
+        if (SEG) return bload(make_rsrc(a.mask, g.dim_y * 2u * plane_b), r * 2u * plane_b + c.wb, (uint32_t)b * plane_b);
+        return bload(make_rsrc(a.mask + (size_t)r * 2 * g.P, 2u * plane_b), c.wb, (uint32_t)b * plane_b);
+    };
+    const int dy = (int)g.dim_y;
+    auto wrap = [&](int r) -> uint32_t {
+        if (r < 0) {
+            r += dy;
+            if (r < 0) r = (r % dy + dy) % dy;
+        } else if (r >= dy) {
+            r -= dy;
+            if (r >= dy) r %= dy;
+        }
+        return (uint32_t)r;
+    };
+
+    // Wave geometry for the uniform mask / force tests: does the wave (words
+    // [wlo, wlo + 64) modulo W) touch words [lo, hi)?
+    const int wlo = (int)(strip * (64u - 2u * a.halo)) - (int)a.halo;  // strip mode only
+    const int whi = wlo + 64;
+    auto wave_hits = [&](int lo, int hi) -> bool {
+        if (lo >= hi) return false;
+        if (g.W < 64) return true;  // the wave wraps the whole row
+        const int W = (int)g.W;
+        auto hits = [&](int a0, int a1) { return a0 < hi && a1 > lo; };
+        if (wlo < 0) return hits(0, whi) || hits(W + wlo, W);
+        if (whi > W) return hits(wlo, W) || hits(0, whi - W);
+        return hits(wlo, whi);
+    };
+    // Strip mode: which input rows of the walk hold solids in this wave's
+    // words, 64 iterations at a time.  Lane l tests the row of iteration
+    // i0 + l against the row's range of non-fluid words (one vector load per
+    // bound), and the ballot gives bit l.  A scalar load per row instead would
+    // be waited for before each row's branch, and at the start of a launch
+    // every wave waits at once.
+    auto solid_rows = [&](uint32_t i0) -> uint64_t {
+        bool hit = false;
+        const uint32_t i = i0 + lane;
+        if (i < n_iter) {
+            const uint32_t r = wrap((int)(y0 + i) - K);
+            hit = wave_hits(a.row_range[r], a.row_range[g.dim_y + r]);
+        }
+        return __builtin_amdgcn_ballot_w64(hit);
+    };
+    const bool force_cols = a.fdec && wave_hits((int)a.fw0, (int)(a.fw0 + a.fwn));
+    const bool force_mine = a.fdec && c.w >= a.fw0 && c.w < a.fw0 + a.fwn;
+    const uint32_t fvoff = force_mine ? (c.w - a.fw0) * 4u : kOutside;  // strip mode
+    // K <= 2: the walk's first input row is loaded before the tests below, so
+    // that their loads wait out one memory latency together with these (short
+    // walks on small lattices feel it).  At K = 4 the registers this holds
+    // across the tests push the walks into spilling, so the walk loads it.
+    constexpr bool kEarlyRow = K <= 2;
+    const uint32_t row_first = wrap((int)y0 - K);
+    uint32_t first[7];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) first[d] = kEarlyRow ? load_plane(row_first, d) : 0u;
+
+    // Decide once per tile whether the mask and force paths can be needed.
+    // srows holds the solid rows of iterations 0..63 (strip mode).
+    bool tile_solids = false, tile_force = false;
+    uint64_t srows = 0;
+    if (SEG) {
+        // Segments walk consecutive row tiles, so rows [wy0 - K, wy0 + S th
+        // + K) cover every row the live ones read; a wave with no solid row
+        // there takes the lean walk.  Forcing: whenever the lattice has it
+        // (rows differ per segment).
+        tile_force = a.fdec != nullptr;
+        if (a.mask) {
+            const uint32_t S = 64u / g.W;
+            const uint32_t wy0 = tile * S * a.th, span = S * a.th + 2u * (uint32_t)K;
+            for (uint32_t i0 = 0; !tile_solids && i0 < span; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                bool hit = false;
+                if (i < span) {
+                    const uint32_t r = wrap((int)(wy0 + i) - K);
+                    hit = a.row_range[r] < a.row_range[g.dim_y + r];
+                }
+                tile_solids = __builtin_amdgcn_ballot_w64(hit) != 0;
+            }
+        }
+    } else {
+        if (a.mask) {
+            srows = solid_rows(0);
+            tile_solids = srows != 0;
+            for (uint32_t i0 = 64; !tile_solids && i0 < n_iter; i0 += 64) tile_solids = solid_rows(i0) != 0;
+        }
+        for (uint32_t i0 = 0; force_cols && !tile_force && i0 < n_iter; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            bool f = false;
+            if (i < n_iter) {
+                const uint32_t r = wrap((int)(y0 + i) - K);
+                f = r >= a.fy0 && r < a.fy1;
+            }
+            tile_force = __builtin_amdgcn_ballot_w64(f) != 0;
+        }
+    }
+
+    auto walk = [&](auto side_tag) {
+    constexpr bool SIDE = decltype(side_tag)::value;
+    const bool t_solids = SIDE && tile_solids, t_force = SIDE && tile_force;
+    // pipeline windows between level b and b+1: rows j-1 (planes 1, 2) and j
+    uint32_t winA[K][7];
+    uint32_t winB[K][2];
+    uint32_t dl[K + 1][4];            // chirality delay lines
+    uint32_t mns[K + 1], msl[K + 1];  // mask delay line: [s] = masks of row i - s
+    uint32_t fnext[K + 1];            // force decisions for the next iteration's levels
+    uint32_t callw[K + 1][4];         // chirality calls of the current iteration
+#pragma unroll
+    for (int s = 0; s <= K; ++s) {
+        mns[s] = msl[s] = fnext[s] = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) dl[s][k] = callw[s][k] = 0;
+    }
+#pragma unroll
+    for (int b = 0; b < K; ++b) {
+#pragma unroll
+        for (int d = 0; d < 7; ++d) winA[b][d] = 0;
+        winB[b][0] = winB[b][1] = 0;
+    }
+
+    // rows: row_i is the input row of the current iteration; lrow[s] the row
+    // level s works on (row_i - s on the torus)
+    uint32_t row_i = row_first;
+    uint32_t lrow[K + 1];
+#pragma unroll
+    for (int s = 1; s <= K; ++s) lrow[s] = wrap((int)row_i - s);
+
+    uint32_t nxt[7];
+    uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
+    {
+        if (t_solids && (SEG || (srows & 1u))) {
+            nns = load_mask(row_i, 0);
+            nsl = load_mask(row_i, 1);
+        }
+#pragma unroll
+        for (int d = 0; d < Q; ++d) nxt[d] = kEarlyRow ? first[d] : load_plane(row_i, d);
+    }
+
+    // One step of the row walk.  STEADY iterations (i >= 2K) have every level
+    // active.  PAR is i & 1: with even tile starts and an even dim_y (FHP), the
+    // row of level s has parity (K + PAR + s) & 1.
+    auto iterate = [&](auto steady_tag, auto par_tag, uint32_t i) {
+        constexpr bool STEADY = decltype(steady_tag)::value;
+        constexpr int PAR = decltype(par_tag)::value;
+        const uint32_t row_next = row_i + 1 == g.dim_y ? 0u : row_i + 1;
+        uint32_t cur[7];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) cur[d] = nxt[d];
+        mns[0] = nns;
+        msl[0] = nsl;
+        uint32_t fcur[K + 1];
+#pragma unroll
+        for (int s = 1; s <= K; ++s) fcur[s] = fnext[s];
+
+        // side loads for the next iteration, then the next row's planes
+        if (i + 1 < n_iter) {
+            const uint32_t rn = row_next;
+            nns = nsl = 0;
+            // the row of iteration i + 1; a new group of 64 is tested on entry
+            if (t_solids && !SEG && ((i + 1u) & 63u) == 0) srows = solid_rows(i + 1u);
+            if (t_solids && (SEG || ((srows >> ((i + 1u) & 63u)) & 1u))) {
+                nns = load_mask(rn, 0);
+                nsl = load_mask(rn, 1);
+            }
+            if (t_force) {
+                // one buffer load per level on every lane, reading 0 where
+                // there is no decision
+#pragma unroll
+                for (int s = 1; s <= K; ++s) {
+                    // next iteration, level s works on row_i - (s - 1)
+                    const uint32_t jn = s == 1 ? row_i : lrow[s - 1];
+                    const bool lev = STEADY || i + 1 >= (uint32_t)(2 * s);
+                    const uint32_t* base = a.fdec + (size_t)(s - 1) * a.fdec_stride;
+                    if (SEG) {  // rows differ per lane
+                        const bool ok = lev && jn >= a.fy0 && jn < a.fy1 && force_mine;
+                        const uint32_t voff = ok ? ((jn - a.fy0) * a.fwn + (c.w - a.fw0)) * 4u : kOutside;
+                        fnext[s] = bload(make_rsrc(base, fplane_b), voff, 0);
+                    } else {    // the row is uniform: its offset is scalar, an
+                                // empty descriptor reads 0 for rows outside
+                        const bool ok = lev && jn >= a.fy0 && jn < a.fy1;
+                        fnext[s] = bload(make_rsrc(base, ok ? fplane_b : 0u), fvoff,
+                                         ok ? (jn - a.fy0) * a.fwn * 4u : 0u);
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < Q; ++d) nxt[d] = load_plane(rn, d);
+        }
+
+#pragma unroll
+        for (int s = 1; s <= K; ++s) {
+            uint32_t out[7];
+            const bool active = STEADY || i >= (uint32_t)(2 * s);          // wave-uniform
+            const bool have_prev = STEADY || i >= (uint32_t)(2 * (s - 1));
+            const uint32_t j = lrow[s];                      // row of level s
+            const uint32_t t = a.t0 + (uint32_t)(s - 1);     // its step
+            const bool odd_row = KIND != kHPP && (((K + PAR + s) & 1) != 0);
+            if (active) {
+                // pull stream (src/omp_lattice.cpp:141-180)
+                uint32_t in[7];
+                const uint32_t* A = winA[s - 1];
+                const uint32_t* B = winB[s - 1];
+                if (KIND == kHPP) {
+                    in[0] = shift_m1<RAGGED, SEG>(A[0], c);            // from (x-1, j)
+                    in[1] = B[0];                         // from (x, j-1), plane 1 kept in B[0]
+                    in[2] = shift_p1<RAGGED, SEG>(A[2], c);    // from (x+1, j)
+                    in[3] = cur[3];                       // from (x, j+1)
+                } else {
+                    in[0] = shift_m1<RAGGED, SEG>(A[0], c);
+                    in[3] = shift_p1<RAGGED, SEG>(A[3], c);
+                    if (Q == 7) in[6] = A[6];
+                    if (odd_row) {
+                        in[1] = B[0];
+                        in[2] = shift_p1<RAGGED, SEG>(B[1], c);
+                        in[4] = shift_p1<RAGGED, SEG>(cur[4], c);
+                        in[5] = cur[5];
+                    } else {
+                        in[1] = shift_m1<RAGGED, SEG>(B[0], c);
+                        in[2] = B[1];
+                        in[4] = cur[4];
+                        in[5] = shift_m1<RAGGED, SEG>(cur[5], c);
+                    }
+                }
+                uint32_t st[7];
+#pragma unroll
+                for (int d = 0; d < Q; ++d) st[d] = in[d];
+                uint32_t p = 0;
+                if (KIND != kHPP) {
+                    if (chir_caller(O, s)) {
+                        const U4 r = philox4x32_10(c.w, j, t >> 2, 0u, a.key0, a.key1);
+                        callw[s][0] = r.x; callw[s][1] = r.y; callw[s][2] = r.z; callw[s][3] = r.w;
+                        p = pick4(r, K == 1 ? (t & 3u) : (uint32_t)chir_word(O, s));
+                    } else {
+                        p = dl[s][chir_delay(O, s) - 1];
+                    }
+                }
+                collide<KIND, uint32_t>(st, p);
+                if (t_solids) {
+                    const uint32_t ns = mns[s], sl = msl[s];
+                    if (__builtin_amdgcn_ballot_w64((ns | sl) != 0) != 0)
+                        apply_solids<KIND, uint32_t>(st, in, ns, sl, edge, j == 0 || j + 1 == g.dim_y);
+                    // fcur is zero outside the forced rows
+                    if (t_force && (SEG || (j >= a.fy0 && j < a.fy1)))
+                        apply_force<KIND, uint32_t>(st, fcur[s] & ~(ns | sl), a.force_dir);
+                } else if (t_force && (SEG || (j >= a.fy0 && j < a.fy1))) {
+                    apply_force<KIND, uint32_t>(st, fcur[s], a.force_dir);
+                }
+#pragma unroll
+                for (int d = 0; d < Q; ++d) out[d] = RAGGED ? (st[d] & c.vmask) : st[d];
+            }
+            // slide the window of boundary s-1 (its newest row is cur)
+            if (have_prev) {
+                if (KIND == kHPP) {
+                    winB[s - 1][0] = winA[s - 1][1];
+                } else {
+                    winB[s - 1][0] = winA[s - 1][1];
+                    winB[s - 1][1] = winA[s - 1][2];
+                }
+#pragma unroll
+                for (int d = 0; d < Q; ++d) winA[s - 1][d] = cur[d];
+            }
+            if (active) {
+#pragma unroll
+                for (int d = 0; d < Q; ++d) cur[d] = out[d];
+                // segment mode: rows past a segment's own tile belong to the next
+                if (s == K) {
+                    const bool wr = writer && (!SEG || i - (uint32_t)(2 * K) < rows);
+#pragma unroll
+                    for (int d = 0; d < Q; ++d) store_plane(j, d, cur[d], wr);
+                }
+            }
+        }
+
+        // mask delay line: level s next iteration handles the row level s-1 had
+        if (t_solids) {
+#pragma unroll
+            for (int s = K; s >= 1; --s) {
+                mns[s] = mns[s - 1];
+                msl[s] = msl[s - 1];
+            }
+        }
+
+        // Advance the chirality delay lines.  A level s that makes no call
+        // uses word chir_word of the call level chir_src made for the same row
+        // D = chir_delay iterations earlier: dl[s][0..D-1] is that queue,
+        // dl[s][D-1] the oldest entry.
+        if (KIND != kHPP) {
+#pragma unroll
+            for (int s = 2; s <= K; ++s) {
+                const int D = chir_delay(O, s);
+                if (D == 0) continue;
+#pragma unroll
+                for (int k = D - 1; k >= 1; --k) dl[s][k] = dl[s][k - 1];
+                dl[s][0] = callw[chir_src(O, s)][chir_word(O, s)];
+            }
+        }
+        // advance the rows
+#pragma unroll
+        for (int s = K; s >= 2; --s) lrow[s] = lrow[s - 1];
+        lrow[1] = row_i;
+        row_i = row_next;
+    };
+
+    using F = std::false_type;
+    using T = std::true_type;
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    uint32_t i = 0;
+    for (; i < (uint32_t)(2 * K); i += 2) {   // pipeline fill (n_iter > 2K)
+        iterate(F{}, P0{}, i);
+        iterate(F{}, P1{}, i + 1);
+    }
+    for (; i + 1 < n_iter; i += 2) {
+        iterate(T{}, P0{}, i);
+        iterate(T{}, P1{}, i + 1);
+    }
+    if (i < n_iter) iterate(T{}, P0{}, i);
+    };  // walk
+
+    // Most tiles of a large lattice are all fluid and unforced: they take a
+    // copy of the walk with that code left out (no tests, no branches).  Rows
+    // with a short last word keep the one walk: the second copy would push
+    // their kernels past the register budget.
+    if (RAGGED || tile_solids || tile_force) walk(std::true_type{});
+    else walk(std::false_type{});
+}
+
+// Strip-mode block order (TileOrder): a block holds four consecutive strips
+// of one row tile; block columns are those four-strip groups.  Waves whose
+// strips carry forcing run longer, and dispatched in plain order the last of
+// them trail the rest of the grid; so the heavy block columns (of every row
+// tile) are dispatched first and the light ones fill in behind them.
+template <int KIND, int K, int O, bool RAGGED, bool SEG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+void tile_kernel(TileArgs a, TileOrder o)
+{
+    // segment mode: grid (tile groups / 4); strip mode: grid (nbx * ntiles)
+    if (SEG) {
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+        tile_body<KIND, K, O, RAGGED, true>(a, 0, wave);
+        return;
+    }
+    // Workgroups go to the 8 XCDs round-robin.  Each XCD takes a contiguous
+    // run of the order instead, so the tiles it holds are vertical neighbours
+    // whose halo rows meet in its L2 (32768 x 8192 Karman: 6830 -> 6990
+    // GSUP/s at the best tile height of each mapping).  The heavy and the
+    // light blocks are mapped apart, so that every XCD gets its share of both
+    // (7086 -> 7452: mapped as one range, the forced blocks all landed on the
+    // first XCDs, which then finished last).
+    const uint32_t H = o.nheavy * o.ntiles;
+    const bool in_heavy = blockIdx.x < H;
+    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - H, n = in_heavy ? H : gridDim.x - H;
+    const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
+    const uint32_t L = (in_heavy ? 0u : H) + xcd * q + min(xcd, r) + (b >> 3);
+    uint32_t bx, tile;
+    if (L < H) {
+        tile = L / o.nheavy;
+        bx = o.heavy[L - tile * o.nheavy];
+    } else {
+        const uint32_t nl = o.nbx - o.nheavy, l = L - H;
+        tile = l / nl;
+        bx = l - tile * nl;  // the bx-th light column: step over the heavy ones
+        for (uint32_t h = 0; h < o.nheavy; ++h)
+            if (o.heavy[h] <= bx) ++bx;
+    }
+    const uint32_t strip = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
+    tile_body<KIND, K, O, RAGGED, false>(a, strip, tile);
+}
+
+// Batched form: every wave takes one work item (job, strip, tile, shape) and
+// runs that job's lattice.  A job's step, current buffer and force-window
+// origin live in device memory (the job's last wave of a round moves them on),
+// so queued rounds need no host-to-device traffic and no launch between them.  One launch covers the
+// items of one (model, K, ragged) class; the top bit of an item's shape picks
+// segment mode for its job, so strip and segment jobs share the launch.
+template <int KIND, int K, bool RAGGED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
+                       const WorkItem* __restrict__ items, uint32_t n_items, uint32_t heavy_blocks)
+{
+    // Blocks go to the 8 XCDs round-robin; the leading (heavy) blocks and the
+    // rest each map to contiguous runs per XCD, as in tile_kernel.
+    const bool in_heavy = blockIdx.x < heavy_blocks;
+    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - heavy_blocks;
+    const uint32_t n = in_heavy ? heavy_blocks : gridDim.x - heavy_blocks;
+    const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
+    const uint32_t blk = (in_heavy ? 0u : heavy_blocks) + xcd * q + min(xcd, r) + (b >> 3);
+    // wave-uniform by construction; readfirstlane lets the compiler keep the
+    // job record in scalar registers
+    const uint32_t wi = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
+    if (wi >= n_items) return;
+    const WorkItem it = items[wi];
+    const BatchState bs = state[it.job];
+    TileArgs a = jobs[it.job].a;
+    a.t0 = bs.step;
+    a.th = it.shape & 0xFFFFu;
+    a.halo = (it.shape >> 16) & 0x7FFFu;
+    a.src = bs.cur ? jobs[it.job].buf1 : jobs[it.job].buf0;
+    a.dst = bs.cur ? jobs[it.job].buf0 : jobs[it.job].buf1;
+    if (a.fdec) a.fdec += (size_t)(bs.step - bs.fdec_step0) * a.fdec_stride;
+    // batch rounds start K-step passes on multiples of 4 (O = 0)
+    if (it.shape >> 31) tile_body<KIND, K, 0, RAGGED, true>(a, it.strip, it.tile);
+    else tile_body<KIND, K, 0, RAGGED, false>(a, it.strip, it.tile);
+    // The job's last wave of the round advances it.  Every other wave of the
+    // job has read the state and counted itself, so nothing in this launch
+    // reads the state after the update, and the next launch on the stream
+    // sees it.
+    if ((threadIdx.x & 63u) == 0) {
+        BatchState& st = state[it.job];
+        if (atomicAdd(&st.done, 1u) + 1u == it.job_items) {
+            st.done = 0;
+            st.step = bs.step + (uint32_t)K;
+            st.cur = bs.cur ^ 1u;
+        }
+    }
+}
+
+// Launchers of one (model, K) pair; lgca_tile.hip has validated the shape and
+// built the grid.  The pass offset O = t0 & 3 selects the instantiation.
+template <int KIND, int K>
+hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblocks, bool ragged,
+                          bool seg, hipStream_t s)
+{
+    const int off = K == 1 ? 0 : (int)(a.t0 & 3u);
+    const dim3 grid(nblocks), block(256);
+#define LGCA_TILE_LAUNCH(OO, RG, SG) \
+    hipLaunchKernelGGL((tile_kernel<KIND, K, OO, RG, SG>), grid, block, 0, s, a, o)
+#define LGCA_TILE_OFF(OO)                                         \
+    if (off == OO) {                                              \
+        if (ragged) {                                             \
+            if (seg) LGCA_TILE_LAUNCH(OO, true, true);            \
+            else LGCA_TILE_LAUNCH(OO, true, false);               \
+        } else {                                                  \
+            if (seg) LGCA_TILE_LAUNCH(OO, false, true);           \
+            else LGCA_TILE_LAUNCH(OO, false, false);              \
+        }                                                         \
+        return hipGetLastError();                                 \
+    }
+    LGCA_TILE_OFF(0)
+    if constexpr (K > 1) {
+        LGCA_TILE_OFF(1)
+        LGCA_TILE_OFF(2)
+        LGCA_TILE_OFF(3)
+    }
+#undef LGCA_TILE_OFF
+#undef LGCA_TILE_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+template <int KIND, int K>
+hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
+                                uint32_t n_items, uint32_t nblocks, uint32_t heavy_blocks, bool ragged,
+                                hipStream_t s)
+{
+    if (ragged)
+        hipLaunchKernelGGL((tile_batch_kernel<KIND, K, true>), dim3(nblocks), dim3(256), 0, s, jobs, state,
+                           items, n_items, heavy_blocks);
+    else
+        hipLaunchKernelGGL((tile_batch_kernel<KIND, K, false>), dim3(nblocks), dim3(256), 0, s, jobs, state,
+                           items, n_items, heavy_blocks);
+    return hipGetLastError();
+}
+
+}  // namespace lgca

@@ -271,6 +271,13 @@ public:
         check(lgca_set_forcing(dev(), dir == 'x' ? LGCA_FORCE_X : (dir == 'y' ? LGCA_FORCE_Y : LGCA_FORCE_NONE),
                                p, x0, x1, y0, y1));
     }
+    // Times the candidate launch shapes of lgca_step now, without changing the
+    // state or the step, and keeps the fastest (lgca_tune).
+    void tune()
+    {
+        sync_types();
+        check(lgca_tune(dev()));
+    }
     // The viewer's forcing controller on the GPU: a tick decides on the
     // snapshot's mean velocity, applies the count-mode force, runs `steps`
     // steps and snapshots, without the host waiting (lgca_controller_*).

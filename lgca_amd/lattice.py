@@ -242,6 +242,12 @@ class Lattice:
         (0 = automatic)."""
         self._check(lib().lgca_set_tuning(self._h, int(steps_per_pass), int(tile_rows)))
 
+    def tune(self) -> None:
+        """Times the candidate launch shapes now (without changing the state or
+        the step) and keeps the fastest; see lgca_tune."""
+        self.copy_data_to_device()
+        self._check(lib().lgca_tune(self._h))
+
     def step(self, nsteps: int = 1) -> None:
         self.copy_data_to_device()
         self._check(lib().lgca_step(self._h, int(nsteps)))

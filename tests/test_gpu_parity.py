@@ -182,11 +182,12 @@ def test_config_hpp_256_periodic_1000_steps(Lattice):
     lat.close()
 
 
-def test_config_fhp2_pipe_truncated(Lattice):
+def test_config_fhp2_pipe_20000_steps(Lattice):
     """FHP-II 2048x512 pipe, walls top and bottom, fill 0.3, per-cell x force
-    p = 1e-3, seed 7, r = 16 (32x32 blocks); 300 of the 20000 steps."""
-    say("[config] FHP-II 2048x512 pipe x 300")
-    dx, dy = 2048, 512
+    p = 1e-3, seed 7, r = 16 (32x32 blocks), all 20000 steps through the
+    default launch path (trial of launch shapes included)."""
+    say("[config] FHP-II 2048x512 pipe x 20000")
+    dx, dy, steps = 2048, 512, 20000
     lat = Lattice(2, dim_x=dx, dim_y=dy, coarse_graining_radius=16, seed=7)
     lat.apply_bc_pipe()
     lat.init_random(0.3)
@@ -195,15 +196,21 @@ def test_config_fhp2_pipe_truncated(Lattice):
     np.testing.assert_array_equal(lat.get_state_u8(), state)
     n0 = lat.get_n_particles()
     lat.set_forcing("x", 1e-3)
-    lat.step(300)
-    want = orc.run(2, dx, dy, types, state, 7, 0, 300, dict(dir="x", p=1e-3), nthreads=8)
+    # in chunks, as a viewer steps: the trial finishes early and later chunks
+    # run the kept shape
+    for n in (300, 700, 9000, 10000):
+        lat.step(n)
+    want = orc.run(2, dx, dy, types, state, 7, 0, steps, dict(dir="x", p=1e-3), nthreads=16)
     np.testing.assert_array_equal(lat.get_state_u8(), want)
+    assert lat.steps == steps
     assert lat.get_n_particles() == n0
     lat.copy_data_to_output_buffer()
     lat.post_process(cells=False)
     _, _, md, mm = orc.post_process(2, dx, dy, 16, want)
     np.testing.assert_array_equal(lat.mean_density().view(np.uint32), md.view(np.uint32))
     np.testing.assert_array_equal(lat.mean_momentum().view(np.uint32), mm.view(np.uint32))
+    info = lat.info()
+    say(f"[config] kept steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
     lat.close()
 
 
@@ -337,11 +344,13 @@ def test_codec_round_trip_ragged(Lattice):
         lat.close()
 
 
-def test_conservation_at_full_karman_size(Lattice):
+def test_full_karman_size_through_the_default_path(Lattice):
     """FHP-III 32768x8192 with the cylinder of diameter 512 at (4096, 4099),
-    walls, fill 0.25, x force p = 2e-3 on x < 64, seed 11: particle count is
-    conserved over 50 steps, and the first 2 steps match the oracle."""
-    dx, dy = 32768, 8192
+    walls, fill 0.25, x force p = 2e-3 on x < 64, seed 11, r = 32: 96 steps
+    through the default launch path (the trial's 72 steps of candidate shapes,
+    then the kept shape) equal the oracle's bits, and so do the coarse fields
+    of the final state."""
+    dx, dy, steps = 32768, 8192, 96
     say("[full] karman 32768x8192")
     lat = Lattice(3, dim_x=dx, dim_y=dy, coarse_graining_radius=32, seed=11)
     lat.apply_bc_cylinder(4096, 4099, 512.0)
@@ -353,12 +362,73 @@ def test_conservation_at_full_karman_size(Lattice):
     np.testing.assert_array_equal(state, want0)
     del want0
     n0 = lat.get_n_particles()
-    lat.step(2)
+    lat.step(76)
+    lat.synchronize()
+    lat.step(20)  # decides on the trial, then runs the kept shape
+    info = lat.info()
+    say(f"[full] kept steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
+    assert info.steps_per_pass == 4 and info.tile_rows > 0
     got = lat.get_state_u8()
-    want = orc.run(3, dx, dy, types, state, 11, 0, 2, dict(dir="x", p=2e-3, x0=0, x1=64),
-                   nthreads=16)
-    np.testing.assert_array_equal(got, want)
-    del got, want, state
-    lat.step(48)
     assert lat.get_n_particles() == n0
+    want = orc.run(3, dx, dy, types, state, 11, 0, steps, dict(dir="x", p=2e-3, x0=0, x1=64),
+                   nthreads=16)
+    del state
+    np.testing.assert_array_equal(got, want)
+    del got
+    lat.copy_data_to_output_buffer()
+    lat.post_process(cells=False)
+    _, _, md, mm = orc.post_process(3, dx, dy, 32, want)
+    np.testing.assert_array_equal(lat.mean_density().view(np.uint32), md.view(np.uint32))
+    np.testing.assert_array_equal(lat.mean_momentum().view(np.uint32), mm.view(np.uint32))
+    lat.close()
+
+
+@pytest.mark.parametrize("k", [2, 4, 8])
+@pytest.mark.parametrize("model,dx,dy,geom", [(0, 700, 40, "cylinder"), (1, 96, 30, "mixed"),
+                                              (2, 3000, 26, "mixed"), (3, 2048, 20, "cylinder")])
+def test_passes_start_at_every_step_offset(Lattice, model, dx, dy, geom, k):
+    """K-step passes started at steps 4m + o for o = 0..3 (the chirality group
+    offset the kernels are compiled for), each run as one launch of k steps,
+    with forcing on part of the lattice: every offset gives the oracle's bits."""
+    rng = np.random.default_rng(dx + 13 * k + model)
+    types = make_types(rng, dx, dy, geom)
+    state = random_state(rng, model, dx * dy)
+    force = dict(dir="x", p=0.2, x0=dx // 4, x1=dx // 2)
+    for off in range(4):
+        step0 = 40 + off
+        say(f"[offset] model={model} {dx}x{dy} k={k} t0={step0}")
+        _, got, want = run_both(Lattice, model, dx, dy, types, state, seed=99 + off, step0=step0,
+                                nsteps=k, force=force, tuning=(k, 4))
+        np.testing.assert_array_equal(got, want)
+
+
+def test_tune_keeps_state_and_step(Lattice):
+    """lgca_tune times candidate passes without changing the state or the
+    step, reports a candidate shape, and later steps give the oracle's bits."""
+    dx, dy, seed = 4096, 512, 21
+    rng = np.random.default_rng(seed)
+    types = make_types(rng, dx, dy, "cylinder")
+    state = random_state(rng, 2, dx * dy)
+    force = dict(dir="x", p=0.01, x0=0, x1=128)
+    lat = Lattice(2, dim_x=dx, dim_y=dy, coarse_graining_radius=1, seed=seed)
+    lat.set_cell_types(types)
+    lat.set_state_u8(state)
+    lat.set_step(5)
+    lat.set_forcing("x", 0.01, 0, 128)
+    say("[tune] 4096x512 FHP-II, cylinder, forced strip")
+    lat.tune()
+    info = lat.info()
+    say(f"[tune] kept steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
+    assert info.steps_per_pass in (2, 4) and info.tile_rows in {2, 4, 8, 12, 16, 20, 24, 32}
+    assert lat.steps == 5
+    np.testing.assert_array_equal(lat.get_state_u8(), state)
+    lat.step(37)
+    want = orc.run(2, dx, dy, types, state, seed, 5, 37, force, nthreads=8)
+    np.testing.assert_array_equal(lat.get_state_u8(), want)
+    assert (lat.info().steps_per_pass, lat.info().tile_rows) == (info.steps_per_pass,
+                                                                  info.tile_rows)
+    # a fixed height leaves nothing to tune
+    lat.set_tuning(4, 10)
+    lat.tune()
+    assert lat.info().tile_rows == 10
     lat.close()
