@@ -3,7 +3,7 @@
 // A handle owns, on its GPU: two state buffers (ping-pong bit-planes), the
 // output snapshot, the cell-type masks with a per-row range of non-fluid words,
 // force-decision planes for a window of upcoming steps, post-processing
-// outputs and a small scratch block for counters and validation flags.
+// outputs and a small block for counters and validation flags.
 #include "lgca.h"
 
 #include "lgca_dev.hpp"
@@ -63,7 +63,7 @@ struct lgca_lattice {
     float* velocity = nullptr;
     uint8_t* bytes = nullptr;       // N-byte staging for u8 state / types
     uint32_t* visited = nullptr;    // count-mode force: first-visit bitmap
-    unsigned long long* scratch = nullptr;  // 8 words: counters and flags
+    unsigned long long* counters = nullptr;  // 16 words: counters and flags
     uint64_t seed = 0, step = 0, force_calls = 0;
     int force_dir = LGCA_FORCE_NONE;
     uint64_t force_thr = 0;
@@ -161,17 +161,17 @@ lgca_status ensure_mask(lgca_lattice* h)
     return LGCA_OK;
 }
 
-lgca_status read_scratch(lgca_lattice* h, unsigned long long* out, int n)
+lgca_status read_counters(lgca_lattice* h, unsigned long long* out, int n)
 {
-    HIPCHK(h, hipMemcpyAsync(out, h->scratch, n * sizeof(unsigned long long),
+    HIPCHK(h, hipMemcpyAsync(out, h->counters, n * sizeof(unsigned long long),
                              hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return LGCA_OK;
 }
 
-lgca_status clear_scratch(lgca_lattice* h)
+lgca_status clear_counters(lgca_lattice* h)
 {
-    HIPCHK(h, launch_fill_u32((uint32_t*)h->scratch, 0u, 16, h->stream));
+    HIPCHK(h, launch_fill_u32((uint32_t*)h->counters, 0u, 16, h->stream));
     return LGCA_OK;
 }
 
@@ -363,8 +363,8 @@ extern "C" lgca_status lgca_create(lgca_lattice** out, int model, uint32_t dim_x
     if ((e = dalloc(&h->state[0], sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(state)"));
     if ((e = dalloc(&h->state[1], sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(state)"));
     if ((e = dalloc(&h->snap, sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(snapshot)"));
-    if ((e = dalloc(&h->scratch, 16 * sizeof(unsigned long long))) != hipSuccess)
-        return bail(hip_fail(h, e, "hipMalloc(scratch)"));
+    if ((e = dalloc(&h->counters, 16 * sizeof(unsigned long long))) != hipSuccess)
+        return bail(hip_fail(h, e, "hipMalloc(counters)"));
     const uint64_t nw = (uint64_t)h->state_words;
     if ((e = launch_fill_u32(h->state[0], 0u, nw, h->stream)) != hipSuccess ||
         (e = launch_fill_u32(h->state[1], 0u, nw, h->stream)) != hipSuccess ||
@@ -384,7 +384,7 @@ extern "C" lgca_status lgca_destroy(lgca_lattice* h)
         if (h->stream && h->stream != h->own_stream) (void)hipStreamSynchronize(h->stream);
         void* ptrs[] = {h->state[0], h->state[1], h->snap, h->mask, h->row_range, h->dec,
                         h->cell_density, h->cell_momentum, h->mean_density, h->mean_momentum,
-                        h->velocity, h->bytes, h->visited, h->scratch, h->ctrl};
+                        h->velocity, h->bytes, h->visited, h->counters, h->ctrl};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t ev : h->trial_ev) (void)hipEventDestroy(ev);
@@ -487,13 +487,13 @@ static lgca_status upload_types(lgca_lattice* h, const uint8_t* types, hipMemcpy
     if (st) return st;
     HIPCHK(h, hipMemcpyAsync(h->bytes, types, n, kind, h->stream));
     if ((st = ensure_mask(h))) return st;
-    if ((st = clear_scratch(h))) return st;
+    if ((st = clear_counters(h))) return st;
     // row ranges start empty: lo[y] = INT_MAX, hi[y] = 0
     HIPCHK(h, launch_fill_u32((uint32_t*)h->row_range, 0x7FFFFFFFu, h->dim_y, h->stream));
     HIPCHK(h, launch_fill_u32((uint32_t*)h->row_range + h->dim_y, 0u, h->dim_y, h->stream));
-    HIPCHK(h, launch_types(h->bytes, h->mask, h->row_range, h->g, (uint32_t*)h->scratch, h->stream));
+    HIPCHK(h, launch_types(h->bytes, h->mask, h->row_range, h->g, (uint32_t*)h->counters, h->stream));
     unsigned long long flag = 0;
-    if ((st = read_scratch(h, &flag, 1))) return st;
+    if ((st = read_counters(h, &flag, 1))) return st;
     if (flag & 1ull) return fail(h, LGCA_ERR_INVALID_STATE, "cell types must be 0, 1 or 2");
     return LGCA_OK;
 }
@@ -528,11 +528,11 @@ static lgca_status upload_state(lgca_lattice* h, const uint8_t* src, hipMemcpyKi
     lgca_status st = ensure_bytes(h);
     if (st) return st;
     HIPCHK(h, hipMemcpyAsync(h->bytes, src, n, kind, h->stream));
-    if ((st = clear_scratch(h))) return st;
+    if ((st = clear_counters(h))) return st;
     HIPCHK(h, launch_fill_u32(h->state[h->cur], 0u, h->state_words, h->stream));
-    HIPCHK(h, launch_pack(h->bytes, h->state[h->cur], h->g, (uint32_t*)h->scratch, h->stream));
+    HIPCHK(h, launch_pack(h->bytes, h->state[h->cur], h->g, (uint32_t*)h->counters, h->stream));
     unsigned long long flag = 0;
-    if ((st = read_scratch(h, &flag, 1))) return st;
+    if ((st = read_counters(h, &flag, 1))) return st;
     if (flag & 1ull) {
         HIPCHK(h, launch_fill_u32(h->state[h->cur], 0u, h->state_words, h->stream));
         return fail(h, LGCA_ERR_INVALID_STATE, "state bytes carry bits at or above num_dir");
@@ -637,13 +637,40 @@ extern "C" lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint
     return LGCA_OK;
 }
 
+// Decision words per step of the force region: its rows x its words.
+static size_t decision_words_per_step(const lgca_lattice* h)
+{
+    const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
+    return (size_t)(fw1 - fw0) * (h->fy1 - h->fy0);
+}
+
+// Queues the force draw of steps [step, step + nsteps) into dec.
+static lgca_status draw_decisions(lgca_lattice* h, uint32_t* dec, uint32_t nsteps)
+{
+    const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
+    ForceDrawArgs fa{};
+    fa.dec = dec;
+    fa.nsteps = nsteps;
+    fa.step0 = (uint32_t)h->step;
+    fa.key0 = (uint32_t)h->seed;
+    fa.key1 = (uint32_t)(h->seed >> 32);
+    fa.q = force_q(h->force_thr);
+    fa.fx0 = h->fx0;
+    fa.fx1 = h->fx1;
+    fa.fy0 = h->fy0;
+    fa.fy1 = h->fy1;
+    fa.fw0 = fw0;
+    fa.fwn = fw1 - fw0;
+    fa.dim_x = h->dim_x;
+    HIPCHK(h, launch_force_draw(fa, h->stream));
+    return LGCA_OK;
+}
+
 // Force decisions are drawn for a window of upcoming steps at once; a window
 // always starts at the current step and covers at least `need` steps.
 static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32_t need)
 {
-    const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
-    const uint32_t fwn = fw1 - fw0, rows = h->fy1 - h->fy0;
-    const size_t per_step = (size_t)fwn * rows;
+    const size_t per_step = decision_words_per_step(h);
     // up to 64 steps, capped at 64 MiB of decision words, never fewer than `need`
     size_t k = (size_t)(64ull << 20) / 4 / per_step;
     if (k > 64) k = 64;
@@ -659,21 +686,8 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32
         HIPCHK(h, dalloc(&h->dec, k * per_step * 4));
         h->dec_cap = k * per_step;
     }
-    ForceDrawArgs fa{};
-    fa.dec = h->dec;
-    fa.nsteps = (uint32_t)k;
-    fa.step0 = (uint32_t)h->step;
-    fa.key0 = (uint32_t)h->seed;
-    fa.key1 = (uint32_t)(h->seed >> 32);
-    fa.q = force_q(h->force_thr);
-    fa.fx0 = h->fx0;
-    fa.fx1 = h->fx1;
-    fa.fy0 = h->fy0;
-    fa.fy1 = h->fy1;
-    fa.fw0 = fw0;
-    fa.fwn = fwn;
-    fa.dim_x = h->dim_x;
-    HIPCHK(h, launch_force_draw(fa, h->stream));
+    lgca_status st = draw_decisions(h, h->dec, (uint32_t)k);
+    if (st) return st;
     h->dec_step0 = h->step;
     h->dec_count = (uint32_t)k;
     return LGCA_OK;
@@ -865,15 +879,18 @@ static TileArgs with_k(TileArgs a, int k)
     return a;
 }
 
-static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps);
+static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_dec = nullptr,
+                               size_t ext_words = 0);
 
 // The launch-shape trial of plan_launch, run at once and waited for: each
 // candidate shape is timed over groups of consecutive passes covering eight
 // steps (three interleaved rounds, the best of each kept), and the fastest
-// per step is kept, the rule's unless another beats it by more than 2 %.
-// Every candidate pass reads the live state and writes the other buffer,
-// which the next step overwrites anyway, so neither the state nor the step
-// changes.
+// per step is kept, the rule's unless another beats it by more than 2 %; the
+// kept shape then makes one untimed pass at each other step offset within a
+// four-step group (the kernels are compiled per offset), so that the steps
+// that follow run warm.  Every pass reads the live state and writes the other
+// buffer, which the next step overwrites anyway, so neither the state nor the
+// step changes.
 constexpr uint32_t kTuneReps = 3;
 extern "C" lgca_status lgca_tune(lgca_lattice* h)
 {
@@ -884,13 +901,17 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
     const std::vector<LaunchShape> cand = trial_candidates(h);
     int kmax = 1;
     for (const LaunchShape& c : cand) kmax = std::max(kmax, c.k);
+    // decisions for the candidates' passes and for the warm-up passes below,
+    // which start up to three steps later
     const uint32_t* fdec = nullptr;
+    const size_t per_step = decision_words_per_step(h);
+    const uint32_t need = (uint32_t)kmax + 3;
     if (h->force_dir != LGCA_FORCE_NONE) {
-        if (h->dec_count == 0 || h->step < h->dec_step0 || h->step + kmax > h->dec_step0 + h->dec_count) {
-            lgca_status st = prepare_decisions(h, 64, (uint32_t)kmax);
+        if (h->dec_count == 0 || h->step < h->dec_step0 || h->step + need > h->dec_step0 + h->dec_count) {
+            lgca_status st = prepare_decisions(h, 64, need);
             if (st) return st;
         }
-        fdec = h->dec + (size_t)(h->step - h->dec_step0) * tile_args(h, 2, nullptr).fdec_stride;
+        fdec = h->dec + (size_t)(h->step - h->dec_step0) * per_step;
     }
     hipEvent_t ev[2] = {nullptr, nullptr};
     lgca_status st = LGCA_OK;
@@ -926,6 +947,15 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
             h->trial_issued = 0;
             h->trial_open = -1;
             h->tuned = cand[pick];
+            // one untimed pass of the kept shape at each later offset within
+            // the four-step group, so that each of its kernels has run once
+            const int k = h->tuned.k;
+            for (uint32_t o = 1; o < 4 && k > 1 && st == LGCA_OK; ++o) {
+                TileArgs a = with_k(tile_args(h, h->tuned.th, fdec ? fdec + o * per_step : nullptr), k);
+                a.t0 += o;
+                fail_hip(launch_tile(h->kind, k, a, h->stream), "launch_tile");
+            }
+            if (st == LGCA_OK) fail_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         }
     }
     for (hipEvent_t e : ev)
@@ -941,22 +971,51 @@ extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
     return step_locked(h, nsteps);
 }
 
-// The body of lgca_step; the caller holds h->mu and the device.
-static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps)
+// The launch shape kept for lgca_step: the fixed rows, else the trial's
+// choice once made, else the rule's.
+static LaunchShape kept_shape(const lgca_lattice* h)
+{
+    const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
+    if (!h->tile_th && h->tuned.th) return h->tuned;
+    return LaunchShape{base_k, auto_tile_rows(h, base_k)};
+}
+
+// The body of lgca_step; the caller holds h->mu and the device.  With
+// ext_dec, force decisions are drawn into those ext_words words (windows of
+// at least one pass, checked by the caller) instead of the handle's own
+// buffer, and launches take the kept shape without trial launches, so that
+// nothing is allocated.
+static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_dec, size_t ext_words)
 {
     const bool forcing = h->force_dir != LGCA_FORCE_NONE;
     const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
-    const size_t per_step = (size_t)(fw1 - fw0) * (h->fy1 - h->fy0);
+    const size_t per_step = decision_words_per_step(h);
+    uint64_t ext_step0 = 0;  // window held in ext_dec
+    uint32_t ext_count = 0;
     uint64_t left = nsteps;
     while (left > 0) {
         int k = 1, ev_start = -1, ev_end = -1;
         uint32_t th = 0;
-        if (h->tile_k != 0) {
+        if (h->tile_k != 0 && ext_dec) {
+            const LaunchShape sh = kept_shape(h);
+            k = steps_for_launch(sh.k, left);
+            th = k == sh.k ? sh.th : auto_tile_rows(h, k);
+        } else if (h->tile_k != 0) {
             lgca_status st = plan_launch(h, left, &k, &th, &ev_start, &ev_end);
             if (st) return st;
         }
         const uint32_t* fdec = nullptr;
-        if (forcing) {
+        if (forcing && ext_dec) {
+            if (ext_count == 0 || h->step + k > ext_step0 + ext_count) {
+                uint64_t kw = std::min<uint64_t>(ext_words / per_step, 64);
+                kw = std::max<uint64_t>(std::min<uint64_t>(kw, left), (uint64_t)k);
+                lgca_status st = draw_decisions(h, ext_dec, (uint32_t)kw);
+                if (st) return st;
+                ext_step0 = h->step;
+                ext_count = (uint32_t)kw;
+            }
+            fdec = ext_dec + (size_t)(h->step - ext_step0) * per_step;
+        } else if (forcing) {
             if (h->dec_count == 0 || h->step < h->dec_step0 ||
                 h->step + k > h->dec_step0 + h->dec_count) {
                 lgca_status st = prepare_decisions(h, left, (uint32_t)k);
@@ -1013,11 +1072,11 @@ extern "C" lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int
     const uint64_t vw = (n + 31) / 32;
     if (!h->visited) HIPCHK(h, dalloc(&h->visited, vw * 4));
     HIPCHK(h, launch_fill_u32(h->visited, 0u, vw, h->stream));
-    lgca_status st = clear_scratch(h);
+    lgca_status st = clear_counters(h);
     if (st) return st;
     const uint32_t call = (uint32_t)h->force_calls++;
     const uint32_t k0 = (uint32_t)h->seed, k1 = (uint32_t)(h->seed >> 32);
-    long long* res = (long long*)h->scratch;
+    long long* res = (long long*)h->counters;
     switch (h->kind) {
     case kHPP: hipLaunchKernelGGL(body_force_count_kernel<kHPP>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
     case kFHP1: hipLaunchKernelGGL(body_force_count_kernel<kFHP1>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
@@ -1025,7 +1084,7 @@ extern "C" lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int
     }
     HIPCHK(h, hipGetLastError());
     unsigned long long v = 0;
-    if ((st = read_scratch(h, &v, 1))) return st;
+    if ((st = read_counters(h, &v, 1))) return st;
     if (reverted) *reverted = (int64_t)v;
     return LGCA_OK;
 }
@@ -1104,11 +1163,11 @@ extern "C" lgca_status lgca_count_particles(lgca_lattice* h, uint64_t* count)
     if (!h || !count) return LGCA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
-    lgca_status st = clear_scratch(h);
+    lgca_status st = clear_counters(h);
     if (st) return st;
-    HIPCHK(h, launch_count(h->state[h->cur], h->state_words, h->scratch, h->stream));
+    HIPCHK(h, launch_count(h->state[h->cur], h->state_words, h->counters, h->stream));
     unsigned long long v = 0;
-    if ((st = read_scratch(h, &v, 1))) return st;
+    if ((st = read_counters(h, &v, 1))) return st;
     *count = v;
     return LGCA_OK;
 }
@@ -1118,11 +1177,11 @@ extern "C" lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2])
     if (!h || !out) return LGCA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
-    lgca_status st = clear_scratch(h);
+    lgca_status st = clear_counters(h);
     if (st) return st;
-    HIPCHK(h, launch_mean_velocity(h->kind, h->snap, h->mask, h->g, (long long*)h->scratch, h->stream));
+    HIPCHK(h, launch_mean_velocity(h->kind, h->snap, h->mask, h->g, (long long*)h->counters, h->stream));
     unsigned long long v[3] = {0, 0, 0};
-    if ((st = read_scratch(h, v, 3))) return st;
+    if ((st = read_counters(h, v, 3))) return st;
     const double sx = (double)(long long)v[0], sy = (double)(long long)v[1];
     const double nf = (double)v[2];
     const double unit_y = h->kind == kHPP ? 1.0 : (double)0.866025388240814208984375f;

@@ -107,7 +107,12 @@ __device__ __forceinline__ uint32_t pick4(const U4& r, uint32_t sel)
 // is made by the group's first level in the pass: level s itself when its
 // word is 0 or s == 1, else level max(1, s - word), which worked on the same
 // row that many iterations earlier.
-__host__ __device__ constexpr int chir_word(int O, int s) { return (O + s - 1) & 3; }
+// O = kRuntimeOffset (batch rounds with K <= 2, whose jobs in one launch may
+// start on either even offset) schedules the calls as for O = 0 and picks the
+// words at run time, which is exact because such a pass starts on an even
+// step: its one or two levels share a group.
+constexpr int kRuntimeOffset = -1;
+__host__ __device__ constexpr int chir_word(int O, int s) { return ((O < 0 ? 0 : O) + s - 1) & 3; }
 __host__ __device__ constexpr bool chir_caller(int O, int s) { return s == 1 || chir_word(O, s) == 0; }
 __host__ __device__ constexpr int chir_src(int O, int s)
 {
@@ -142,7 +147,7 @@ template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 
 // `tile`.  Segment mode (SEG true, rows of W <= 64 words): the wave's lanes
 // form S = 64 / W segments of W lanes, each holding a whole row; segment s
 // walks row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when
-// K == 1, whose one level picks its word at run time).
+// K == 1, whose one level picks its word at run time), or kRuntimeOffset.
 template <int KIND, int K, int O, bool RAGGED, bool SEG>
 __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile)
 {
@@ -446,7 +451,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                     if (chir_caller(O, s)) {
                         const U4 r = philox4x32_10(c.w, j, t >> 2, 0u, a.key0, a.key1);
                         callw[s][0] = r.x; callw[s][1] = r.y; callw[s][2] = r.z; callw[s][3] = r.w;
-                        p = pick4(r, K == 1 ? (t & 3u) : (uint32_t)chir_word(O, s));
+                        p = pick4(r, (K == 1 || O < 0) ? (t & 3u) : (uint32_t)chir_word(O, s));
                     } else {
                         p = dl[s][chir_delay(O, s) - 1];
                     }
@@ -508,7 +513,12 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                 if (D == 0) continue;
 #pragma unroll
                 for (int k = D - 1; k >= 1; --k) dl[s][k] = dl[s][k - 1];
-                dl[s][0] = callw[chir_src(O, s)][chir_word(O, s)];
+                const int c0 = chir_src(O, s);
+                if (O < 0)
+                    dl[s][0] = pick4(U4{callw[c0][0], callw[c0][1], callw[c0][2], callw[c0][3]},
+                                     (a.t0 + (uint32_t)(s - 1)) & 3u);
+                else
+                    dl[s][0] = callw[c0][chir_word(O, s)];
             }
         }
         // advance the rows
@@ -615,9 +625,11 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
     a.src = bs.cur ? jobs[it.job].buf1 : jobs[it.job].buf0;
     a.dst = bs.cur ? jobs[it.job].buf0 : jobs[it.job].buf1;
     if (a.fdec) a.fdec += (size_t)(bs.step - bs.fdec_step0) * a.fdec_stride;
-    // batch rounds start K-step passes on multiples of 4 (O = 0)
-    if (it.shape >> 31) tile_body<KIND, K, 0, RAGGED, true>(a, it.strip, it.tile);
-    else tile_body<KIND, K, 0, RAGGED, false>(a, it.strip, it.tile);
+    // batch rounds start passes of K >= 4 steps on multiples of 4 (O = 0) and
+    // 2-step passes on even steps (picked at run time)
+    constexpr int O = K <= 2 ? kRuntimeOffset : 0;
+    if (it.shape >> 31) tile_body<KIND, K, O, RAGGED, true>(a, it.strip, it.tile);
+    else tile_body<KIND, K, O, RAGGED, false>(a, it.strip, it.tile);
     // The job's last wave of the round advances it.  Every other wave of the
     // job has read the state and counted itself, so nothing in this launch
     // reads the state after the update, and the next launch on the stream
