@@ -136,6 +136,20 @@ LGCA_API lgca_status lgca_set_forcing(lgca_lattice* h, int dir, double p, uint32
  * each fused with the per-cell force when set.  Queued; returns at once. */
 LGCA_API lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps);
 
+/* lgca_step without allocating: the same steps, bits and launch shapes, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference.  lgca_step allocates the handle's force-decision buffer on
+ * demand; this entry point draws the decisions into a fixed array of 2^21
+ * words (8 MiB) that the library holds on each GPU, and runs no launch-shape
+ * trial (it takes the shape a trial or lgca_tune kept, else the rule's).  It
+ * fails with LGCA_ERR_INVALID_ARGUMENT when one pass needs more decision words
+ * than that: (force rows) x (32-cell words of the force columns) x (steps per
+ * pass).  The array is shared by every call on a GPU and keeps the last call's
+ * decisions: calls whose queued work overlaps on one GPU (several handles on
+ * different streams) race; calls on one handle's stream do not. */
+LGCA_API lgca_status lgca_step_noalloc(lgca_lattice* h, uint64_t nsteps);
+
 /* Launch shape of lgca_step: steps fused per pass over HBM (1, 2, 4 or 8; 0
  * selects the one-step kernel that keeps a 128-cell quad per thread; -1, the
  * default, is 4, or 2 where the trial finds 2-step passes faster on a lattice

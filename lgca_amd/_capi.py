@@ -85,6 +85,7 @@ SIGNATURES = {
     "lgca_init_random_masked": [_h, C.c_double, C.c_void_p],
     "lgca_set_forcing": [_h, C.c_int, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32],
     "lgca_step": [_h, C.c_uint64],
+    "lgca_step_noalloc": [_h, C.c_uint64],
     "lgca_set_tuning": [_h, C.c_int, C.c_uint32],
     "lgca_tune": [_h],
     "lgca_apply_body_force_count": [_h, C.c_int, C.c_int64, C.POINTER(C.c_int64)],

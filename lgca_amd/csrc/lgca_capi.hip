@@ -971,6 +971,39 @@ extern "C" lgca_status lgca_step(lgca_lattice* h, uint64_t nsteps)
     return step_locked(h, nsteps);
 }
 
+// ------------------------------------------------- step without allocating
+// lgca_step_noalloc draws its force decisions into `scratch`, a fixed array
+// in device memory, instead of the handle's decision buffer, which lgca_step
+// allocates (and grows) on demand.  Each GPU the library runs on holds its
+// own copy of the array until the process ends.  Host code never passes the
+// array's name to a kernel or copy: on the host it is only a stand-in for
+// the device symbol.  The array's address on each GPU is looked up once, at
+// that GPU's first call, with the GPU current, and kept in g_scratch_dev.
+constexpr size_t kNoAllocScratchWords = (size_t)2 << 20;  // 8 MiB of decision words
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[kMaxScratchDevices];  // per GPU; null until looked up
+}  // namespace
+
+// The device address of `scratch` on the handle's GPU (current when called).
+static lgca_status scratch_address(lgca_lattice* h, uint32_t** out)
+{
+    if (h->device < 0 || h->device >= kMaxScratchDevices)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_step_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[h->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(h, e, "hipGetSymbolAddress(scratch)");
+        g_scratch_dev[h->device] = (uint32_t*)p;
+    }
+    *out = g_scratch_dev[h->device];
+    return LGCA_OK;
+}
+
 // The launch shape kept for lgca_step: the fixed rows, else the trial's
 // choice once made, else the rule's.
 static LaunchShape kept_shape(const lgca_lattice* h)
@@ -978,6 +1011,26 @@ static LaunchShape kept_shape(const lgca_lattice* h)
     const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
     if (!h->tile_th && h->tuned.th) return h->tuned;
     return LaunchShape{base_k, auto_tile_rows(h, base_k)};
+}
+
+extern "C" lgca_status lgca_step_noalloc(lgca_lattice* h, uint64_t nsteps)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    uint32_t* dev_scratch = nullptr;
+    lgca_status st = scratch_address(h, &dev_scratch);
+    if (st) return st;
+    if (h->force_dir != LGCA_FORCE_NONE) {
+        const int k = h->tile_k == 0 ? 1 : kept_shape(h).k;
+        const size_t need = decision_words_per_step(h) * (size_t)k;
+        if (need > kNoAllocScratchWords)
+            return fail(h, LGCA_ERR_INVALID_ARGUMENT,
+                        "lgca_step_noalloc: a " + std::to_string(k) + "-step pass needs " +
+                            std::to_string(need) + " force-decision words; the fixed scratch holds " +
+                            std::to_string(kNoAllocScratchWords));
+    }
+    return step_locked(h, nsteps, dev_scratch, kNoAllocScratchWords);
 }
 
 // The body of lgca_step; the caller holds h->mu and the device.  With
