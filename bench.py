@@ -194,53 +194,6 @@ def other_configs():
 
 # ------------------------------------------------------------ ensembles
 
-def c4_jobs(n: int, seed: int = 2024):
-    """The batched-ensemble workload: FHP-II lattices with long side L in
-    {128..2048} step 64, aspect 1..4 (short side rounded to 64, at least 64),
-    either orientation, periodic / pipe / cylinder (radius 8..L/8 at
-    (dim_x/4, dim_y/2)), fill U(0.1, 0.5), own seed, 1000-10000 steps, r = 16."""
-    import numpy as np
-    from lgca_amd import Job
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for i in range(n):
-        L = int(rng.integers(2, 33)) * 64
-        S = max(64, int(round(L / float(rng.uniform(1.0, 4.0)) / 64.0)) * 64)
-        dx, dy = (L, S) if rng.random() < 0.5 else (S, L)
-        geom = ("periodic", "pipe", "cylinder")[int(rng.integers(0, 3))]
-        rad = int(rng.integers(8, L // 8 + 1))
-        jobs.append(Job("FHP_II", dx, dy, int(rng.integers(1000, 10001)), seed=100000 + i,
-                        coarse_radius=16, fill=float(rng.uniform(0.1, 0.5)), geometry=geom,
-                        cyl_x=dx // 4, cyl_y=dy // 2, cyl_diameter=2.0 * rad))
-    return jobs
-
-
-def c5_jobs(n: int, seed: int = 2025):
-    """The serving workload: model uniform over the four, dim_x in 256..4096
-    and dim_y in 256..1024 (multiples of 64), periodic / pipe with x force
-    p U(1e-4, 5e-3) / cylinder (walls plus a disc of random radius 8..dim_y/8
-    at a random position), fill 0.25, 500-20000 steps, own seed, r = 16."""
-    import numpy as np
-    from lgca_amd import Job
-    rng = np.random.default_rng(seed)
-    jobs = []
-    for i in range(n):
-        model = int(rng.integers(0, 4))
-        dx, dy = int(rng.integers(4, 65)) * 64, int(rng.integers(4, 17)) * 64
-        geom = ("periodic", "pipe", "cylinder")[int(rng.integers(0, 3))]
-        j = Job(model, dx, dy, int(rng.integers(500, 20001)), seed=200000 + i, coarse_radius=16,
-                fill=0.25, geometry=geom)
-        if geom == "pipe":
-            j.force_dir, j.force_p = "x", float(rng.uniform(1e-4, 5e-3))
-        elif geom == "cylinder":
-            rad = int(rng.integers(8, dy // 8 + 1))
-            j.cyl_diameter = 2.0 * rad
-            j.cyl_x = int(rng.integers(rad, dx - rad))
-            j.cyl_y = int(rng.integers(rad, dy - rad))
-        jobs.append(j)
-    return jobs
-
-
 def single_lattice_run(job, want_state: bool = False):
     """One job through the plain single-lattice path, start to finish: create,
     set-up, fill, steps, post-process, results, destroy.  Returns (seconds,
@@ -269,7 +222,8 @@ def single_lattice_run(job, want_state: bool = False):
 def ensemble_bench(n_jobs: int, sample: int):
     import numpy as np
     from lgca_amd import Batch
-    jobs = c4_jobs(n_jobs)
+    from lgca_amd.workloads import ensemble_jobs
+    jobs = ensemble_jobs(n_jobs)
     su = sum(j.dim_x * j.dim_y * j.steps for j in jobs)
     with Batch(hbm_budget=200 * 10 ** 9) as b:
         t0 = time.perf_counter()
@@ -306,13 +260,21 @@ def ensemble_bench(n_jobs: int, sample: int):
 def serving_bench(rates, n_jobs: int, max_active_cells: int):
     import numpy as np
     from lgca_amd import Batch
-    jobs = c5_jobs(n_jobs)
-    rng = np.random.default_rng(7)
-    unit_arrivals = np.cumsum(rng.exponential(1.0, n_jobs))
-    unit_arrivals -= unit_arrivals[0]
-    # one-at-a-time service times, measured once per job
-    service = np.array([single_lattice_run(j)[0] for j in jobs])
+    from lgca_amd.workloads import poisson_arrivals, serving_jobs
+    jobs = serving_jobs(n_jobs)
+    unit_arrivals = poisson_arrivals(n_jobs, 1.0)
+    # one-at-a-time service times, measured once per job; the single-lattice
+    # results of an evenly spaced sample are kept to compare bits with
+    sample = set(range(0, n_jobs, max(1, n_jobs // 8)))
+    service, alone = [], {}
+    for i, j in enumerate(jobs):
+        dt, st, md, mm = single_lattice_run(j, want_state=i in sample)
+        service.append(dt)
+        if i in sample:
+            alone[i] = (st, md, mm)
+    service = np.array(service)
     out = []
+    identical = True
     for lam in rates:
         arr = unit_arrivals / lam
         with Batch(hbm_budget=200 * 10 ** 9) as b:
@@ -326,6 +288,13 @@ def serving_bench(rates, n_jobs: int, max_active_cells: int):
                 ids.append(b.submit(j))
             b.wait(timeout=900)
             infos = [b.info(i) for i in ids]
+            # each sampled job's bits in the mix equal its bits alone
+            for i in sorted(sample):
+                gst, gmd, gmm = b.result(ids[i])
+                st, md, mm = alone[i]
+                identical &= bool(np.array_equal(gst, st)
+                                  and np.array_equal(gmd.view(np.uint32), md.view(np.uint32))
+                                  and np.array_equal(gmm.view(np.uint32), mm.view(np.uint32)))
         lat = np.array([x["done_s"] - x["submit_s"] for x in infos])
         span = max(x["done_s"] for x in infos) - min(x["submit_s"] for x in infos)
         # one at a time: the same arrivals served first-come first-served
@@ -346,6 +315,7 @@ def serving_bench(rates, n_jobs: int, max_active_cells: int):
         })
         print(f"[serving] rate {lam}: {out[-1]}", file=sys.stderr, flush=True)
     return {"levels": out, "max_active_cells": max_active_cells,
+            "bits_identical_sampled": identical, "bits_sample": len(sample),
             "mean_service_ms_one_at_a_time": round(float(service.mean()) * 1e3, 2),
             "workload": "Poisson arrivals; HPP/FHP-I/FHP-II/FHP-III, 256x256..4096x1024, "
                         "periodic/pipe(force)/cylinder, 500-20000 steps, 200 GB budget"}

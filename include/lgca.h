@@ -172,7 +172,10 @@ LGCA_API lgca_status lgca_tune(lgca_lattice* h);
  * (src/omp_lattice.cpp:254-346).  Candidate cells come from the hash (stream
  * 3, keyed by a per-handle call counter) instead of rand() % N; the loop stops
  * once `forcing` particles are reverted or 2N candidates are spent, and always
- * makes at least one try.  *reverted receives the count (may be NULL). */
+ * makes at least one try.  `forcing` must fit an int (else
+ * LGCA_ERR_INVALID_ARGUMENT); as in the reference, which compares an unsigned
+ * count with it, a negative amount counts modulo 2^32 and so runs all 2N
+ * candidates.  *reverted receives the count (may be NULL). */
 LGCA_API lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int64_t forcing,
                                         int64_t* reverted);
 
@@ -206,7 +209,8 @@ LGCA_API lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2]);
  * computes it; if it is below u, the forcing becomes the equilibrium amount once the
  * velocity exceeds 0.9 u and is applied as lgca_apply_body_force_count does (same
  * call counter); then `steps` steps and a new snapshot.  Nothing is read back, so
- * ticks queue on the stream while the host post-processes earlier snapshots. */
+ * ticks queue on the stream while the host post-processes earlier snapshots.
+ * Both amounts must lie in [0, 2^31). */
 LGCA_API lgca_status lgca_controller_set(lgca_lattice* h, int dir, float u, int64_t forcing,
                                          int64_t equilibrium_forcing);
 LGCA_API lgca_status lgca_controller_tick(lgca_lattice* h, uint64_t steps);
@@ -216,12 +220,17 @@ LGCA_API lgca_status lgca_controller_get(lgca_lattice* h, int64_t* forcing, int6
                                          float velocity[2]);
 
 /* Checkpoint of the live state: model, sizes, seed, step, per-cell force
- * settings, the count-mode force call counter, cell types and state bytes, in
- * one file with a checksum.  A run resumed from it continues bit for bit as
- * the saved run would have.  No counterpart in the reference, whose state
- * lives only in the process (src/lgca_bitset.h:31-245). */
+ * settings, the count-mode force call counter, cell types, state bytes, the
+ * output snapshot and the forcing controller's state, in one file whose
+ * checksum covers header and payload.  A run resumed from it continues bit for
+ * bit as the saved run would have, post-processing and controller ticks
+ * included.  No counterpart in the reference, whose state lives only in the
+ * process (src/lgca_bitset.h:31-245). */
 LGCA_API lgca_status lgca_save_checkpoint(lgca_lattice* h, const char* path);
-/* Creates a handle on GPU `device` from a checkpoint file. */
+/* Creates a handle on GPU `device` from a checkpoint file.  Damaged files
+ * (LGCA_ERR_INVALID_STATE) are refused; force settings are normalised as
+ * lgca_set_forcing does.  Files of the earlier layout, without a snapshot or
+ * controller state, are read too: their snapshot is the saved state. */
 LGCA_API lgca_status lgca_load_checkpoint(lgca_lattice** out, const char* path, int device);
 
 /* ------------------------------------------------------------------------

@@ -282,6 +282,7 @@ JobLayout job_layout(const lgca_job_desc& d, const JobParams& p, size_t n_halfw)
 
 struct Job {
     uint64_t id = 0;
+    uint32_t readers = 0;  // lgca_batch_result copies in flight (under the batch's mutex)
     lgca_job_desc d{};
     std::vector<uint8_t> types_in, state_in;
     std::vector<int32_t> halfw;
@@ -393,6 +394,7 @@ struct lgca_batch {
     std::mutex mu;
     std::condition_variable cv_work;    // submissions, releases, stop
     std::condition_variable cv_done;    // completions
+    std::condition_variable cv_readers; // a job's result copies finished
     bool stop = false;
     std::string err;                    // sticky worker failure
     std::string call_err;               // last failed client call
@@ -843,8 +845,15 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
 {
     const size_t n = b->active.size();
     b->sig.clear();
+    // The items depend on each job's geometry, slot, k and the tile height.
+    // A job keeps its slot while it runs, but a finished job's slot goes to a
+    // later job of another size: the signature names jobs by their ids, never
+    // reused, so that a new job in an old slot always rebuilds the items.
     b->sig.push_back(th);
-    for (size_t i = 0; i < n; ++i) b->sig.push_back(((uint64_t)b->active[i]->slot << 8) | (uint64_t)ks[i]);
+    for (size_t i = 0; i < n; ++i) {
+        b->sig.push_back(b->active[i]->id);
+        b->sig.push_back(((uint64_t)b->active[i]->slot << 8) | (uint64_t)ks[i]);
+    }
     if (b->sig == b->last_sig) return;
     b->last_sig = b->sig;
     // classes in a fixed order: kind, k, ragged (segment and strip jobs share)
@@ -1439,6 +1448,7 @@ extern "C" lgca_status lgca_batch_result(lgca_batch* b, uint64_t job_id, uint8_t
     char* m = nullptr;
     JobLayout L{};
     size_t n = 0, nc = 0;
+    Job* j = nullptr;
     {
         std::lock_guard<std::mutex> lk(b->mu);
         auto it = b->jobs.find(job_id);
@@ -1446,7 +1456,10 @@ extern "C" lgca_status lgca_batch_result(lgca_batch* b, uint64_t job_id, uint8_t
             b->call_err = "lgca_batch_result: job unknown, not done or released";
             return LGCA_ERR_INVALID_ARGUMENT;
         }
-        const Job* j = it->second.get();
+        j = it->second.get();
+        // lgca_batch_release waits for this count to drop to zero before the
+        // job's memory can go to another job
+        ++j->readers;
         m = j->mem;
         L = j->L;
         n = (size_t)j->d.dim_x * j->d.dim_y;
@@ -1460,6 +1473,10 @@ extern "C" lgca_status lgca_batch_result(lgca_batch* b, uint64_t job_id, uint8_t
     if (mean_momentum && e == hipSuccess)
         e = hipMemcpyAsync(mean_momentum, m + L.mm, nc * 8, hipMemcpyDeviceToHost, b->copy_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->copy_stream);
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        if (--j->readers == 0) b->cv_readers.notify_all();
+    }
     if (e != hipSuccess)
         return set_call_error(b, LGCA_ERR_HIP, std::string("lgca_batch_result: ") + hipGetErrorString(e));
     return LGCA_OK;
@@ -1469,13 +1486,20 @@ extern "C" lgca_status lgca_batch_release(lgca_batch* b, uint64_t job_id)
 {
     if (!b) return LGCA_ERR_INVALID_ARGUMENT;
     {
-        std::lock_guard<std::mutex> lk(b->mu);
+        std::unique_lock<std::mutex> lk(b->mu);
         auto it = b->jobs.find(job_id);
         if (it == b->jobs.end() || it->second->status != LGCA_JOB_DONE) {
             b->call_err = "lgca_batch_release: job unknown, not done or already released";
             return LGCA_ERR_INVALID_ARGUMENT;
         }
         Job* j = it->second.get();
+        // result copies of this job still in flight finish first (jobs stay
+        // in the table for the batch's lifetime, so j stays valid)
+        b->cv_readers.wait(lk, [&] { return j->readers == 0; });
+        if (j->status != LGCA_JOB_DONE) {
+            b->call_err = "lgca_batch_release: job already released";
+            return LGCA_ERR_INVALID_ARGUMENT;
+        }
         b->arena.free(j->mem, j->L.total);
         b->bytes_in_use -= j->L.total;
         j->mem = nullptr;

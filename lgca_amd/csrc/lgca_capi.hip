@@ -10,6 +10,8 @@
 #include "lgca_kernels.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1117,6 +1119,13 @@ extern "C" lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int
                                                    int64_t* reverted)
 {
     if (!h || (dir != LGCA_FORCE_X && dir != LGCA_FORCE_Y)) return LGCA_ERR_INVALID_ARGUMENT;
+    if (forcing < INT32_MIN || forcing > INT32_MAX)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT,
+                    "lgca_apply_body_force_count: forcing must fit an int, as apply_body_force(int) takes it");
+    // The reference compares its unsigned count of reverted particles with the
+    // int amount (src/omp_lattice.cpp:263, :345), so the amount counts modulo
+    // 2^32: a negative one runs all 2N candidates.
+    const long long target = forcing < 0 ? (long long)forcing + 4294967296LL : (long long)forcing;
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
     lgca_status st0 = sync_force_calls(h);
@@ -1131,9 +1140,9 @@ extern "C" lgca_status lgca_apply_body_force_count(lgca_lattice* h, int dir, int
     const uint32_t k0 = (uint32_t)h->seed, k1 = (uint32_t)(h->seed >> 32);
     long long* res = (long long*)h->counters;
     switch (h->kind) {
-    case kHPP: hipLaunchKernelGGL(body_force_count_kernel<kHPP>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
-    case kFHP1: hipLaunchKernelGGL(body_force_count_kernel<kFHP1>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
-    default: hipLaunchKernelGGL(body_force_count_kernel<kFHP2>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, (long long)forcing, h->visited, res); break;
+    case kHPP: hipLaunchKernelGGL(body_force_count_kernel<kHPP>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, target, h->visited, res); break;
+    case kFHP1: hipLaunchKernelGGL(body_force_count_kernel<kFHP1>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, target, h->visited, res); break;
+    default: hipLaunchKernelGGL(body_force_count_kernel<kFHP2>, dim3(1), dim3(64), 0, h->stream, h->state[h->cur], h->mask, h->g, k0, k1, call, dir, target, h->visited, res); break;
     }
     HIPCHK(h, hipGetLastError());
     unsigned long long v = 0;
@@ -1246,28 +1255,42 @@ extern "C" lgca_status lgca_mean_velocity(lgca_lattice* h, float out[2])
 // ------------------------------------------------------------ checkpoints
 // File layout (little-endian), all fixed-size fields first:
 //   "LGCACKP1"  8-byte magic
-//   u32 version (1), i32 model, u32 dim_x, u32 dim_y, u32 coarse_radius, u32 0,
-//   u64 seed, u64 step, u64 force_calls,
+//   u32 version (2; files of version 1 are still read), i32 model,
+//   u32 dim_x, dim_y, coarse_radius, flags (bit 0: controller state below),
+//   u64 seed, step, force_calls,
 //   i32 force_dir, u32 0, u64 force_thr, u32 fx0, fx1, fy0, fy1,
-//   u64 fnv1a64 of the payload,
-//   payload: num_cells cell-type bytes, then num_cells state bytes (bit d =
-//   direction d, the Bitset layout of src/lgca_bitset.h:229-231).
+//   u64 checksum: FNV-1a-64 of the header with this field zero, then of the
+//       payload (version 1: of the payload only),
+//   version 2 only: i32 controller direction, f32 controller u,
+//   i64 equilibrium forcing, current forcing, amount of the last tick, its
+//   reverted particles, f32 mean velocity the last tick saw (x, y);
+//   payload: num_cells cell-type bytes, num_cells state bytes (bit d =
+//   direction d, the Bitset layout of src/lgca_bitset.h:229-231), and in
+//   version 2 num_cells bytes of the output snapshot.
 namespace {
 
 struct CheckpointHeader {
     char magic[8];
     uint32_t version;
     int32_t model;
-    uint32_t dim_x, dim_y, coarse_radius, reserved0;
+    uint32_t dim_x, dim_y, coarse_radius, flags;
     uint64_t seed, step, force_calls;
     int32_t force_dir;
     uint32_t reserved1;
     uint64_t force_thr;
     uint32_t fx0, fx1, fy0, fy1;
     uint64_t checksum;
+    // version 2
+    int32_t ctrl_dir;
+    float ctrl_u;
+    int64_t ctrl_eq, ctrl_forcing, ctrl_amount, ctrl_reverted;
+    float ctrl_v[2];
 };
-static_assert(sizeof(CheckpointHeader) == 96, "checkpoint header layout");
+constexpr size_t kHeaderV1 = 96;
+static_assert(offsetof(CheckpointHeader, ctrl_dir) == kHeaderV1, "checkpoint header layout");
+static_assert(sizeof(CheckpointHeader) == 144, "checkpoint header layout");
 constexpr char kCheckpointMagic[8] = {'L', 'G', 'C', 'A', 'C', 'K', 'P', '1'};
+constexpr uint32_t kCheckpointFlagController = 1;
 
 uint64_t fnv1a64(const uint8_t* p, size_t n, uint64_t h = 0xcbf29ce484222325ull)
 {
@@ -1276,6 +1299,15 @@ uint64_t fnv1a64(const uint8_t* p, size_t n, uint64_t h = 0xcbf29ce484222325ull)
         h *= 0x100000001b3ull;
     }
     return h;
+}
+
+// Version 2 checksum: the header with its checksum field zero, then the payload.
+uint64_t checkpoint_checksum(const CheckpointHeader& hd, const std::string& payload)
+{
+    CheckpointHeader z = hd;
+    z.checksum = 0;
+    const uint64_t h = fnv1a64((const uint8_t*)&z, sizeof z);
+    return fnv1a64((const uint8_t*)payload.data(), payload.size(), h);
 }
 
 }  // namespace
@@ -1288,18 +1320,22 @@ extern "C" lgca_status lgca_save_checkpoint(lgca_lattice* h, const char* path)
     lgca_status st0 = sync_force_calls(h);
     if (st0) return st0;
     const size_t n = (size_t)h->dim_x * h->dim_y;
-    std::string buf(2 * n, '\0');
+    std::string buf(3 * n, '\0');
     uint8_t* types = (uint8_t*)&buf[0];
     uint8_t* state = types + n;
+    uint8_t* snap = state + n;
     lgca_status st = ensure_bytes(h);
     if (st) return st;
     HIPCHK(h, launch_types_unpack(h->mask, h->bytes, h->g, h->stream));
     HIPCHK(h, hipMemcpyAsync(types, h->bytes, n, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, launch_unpack(h->snap, h->bytes, h->g, h->stream));
+    HIPCHK(h, hipMemcpyAsync(snap, h->bytes, n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     if ((st = download_state(h, state, hipMemcpyDeviceToHost))) return st;
     CheckpointHeader hd{};
     std::memcpy(hd.magic, kCheckpointMagic, 8);
-    hd.version = 1;
+    hd.version = 2;
     hd.model = h->model;
     hd.dim_x = h->dim_x;
     hd.dim_y = h->dim_y;
@@ -1313,7 +1349,21 @@ extern "C" lgca_status lgca_save_checkpoint(lgca_lattice* h, const char* path)
     hd.fx1 = h->fx1;
     hd.fy0 = h->fy0;
     hd.fy1 = h->fy1;
-    hd.checksum = fnv1a64((const uint8_t*)buf.data(), buf.size());
+    if (h->ctrl) {
+        Controller c;
+        HIPCHK(h, hipMemcpyAsync(&c, h->ctrl, sizeof c, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        hd.flags |= kCheckpointFlagController;
+        hd.ctrl_dir = h->ctrl_dir;
+        hd.ctrl_u = h->ctrl_u;
+        hd.ctrl_eq = h->ctrl_eq;
+        hd.ctrl_forcing = c.forcing;
+        hd.ctrl_amount = c.amount;
+        hd.ctrl_reverted = c.reverted;
+        hd.ctrl_v[0] = c.v[0];
+        hd.ctrl_v[1] = c.v[1];
+    }
+    hd.checksum = checkpoint_checksum(hd, buf);
     FILE* f = std::fopen(path, "wb");
     if (!f) return fail(h, LGCA_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
     const bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1 &&
@@ -1331,16 +1381,23 @@ extern "C" lgca_status lgca_load_checkpoint(lgca_lattice** out, const char* path
     if (!f) return fail(nullptr, LGCA_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
     CheckpointHeader hd{};
     std::string buf;
-    bool ok = std::fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, kCheckpointMagic, 8) == 0 &&
-              hd.version == 1 && hd.dim_x > 0 && hd.dim_y > 0 &&
-              (uint64_t)hd.dim_x * hd.dim_y < (1ull << 40) && hd.force_dir >= LGCA_FORCE_NONE &&
-              hd.force_dir <= LGCA_FORCE_Y && hd.force_thr <= (1ull << 32) && hd.fx1 <= hd.dim_x &&
-              hd.fy1 <= hd.dim_y;
+    bool ok = std::fread(&hd, kHeaderV1, 1, f) == 1 && std::memcmp(hd.magic, kCheckpointMagic, 8) == 0 &&
+              (hd.version == 1 || hd.version == 2);
+    if (ok && hd.version == 2) ok = std::fread((char*)&hd + kHeaderV1, sizeof hd - kHeaderV1, 1, f) == 1;
+    // sizes and ranges before anything is allocated
+    ok = ok && hd.dim_x > 0 && hd.dim_y > 0 && (uint64_t)hd.dim_x * hd.dim_y < (1ull << 40) &&
+         hd.force_dir >= LGCA_FORCE_NONE && hd.force_dir <= LGCA_FORCE_Y && hd.force_thr <= (1ull << 32);
+    if (ok && hd.version == 1) hd.flags = 0;
+    const bool has_ctrl = ok && (hd.flags & kCheckpointFlagController);
+    if (has_ctrl)
+        ok = (hd.ctrl_dir == LGCA_FORCE_X || hd.ctrl_dir == LGCA_FORCE_Y) && hd.ctrl_eq >= 0 &&
+             hd.ctrl_forcing >= 0 && hd.ctrl_amount >= -1;
     if (ok) {
         const size_t n = (size_t)hd.dim_x * hd.dim_y;
-        buf.resize(2 * n);
+        buf.resize((hd.version == 2 ? 3 : 2) * n);
         ok = std::fread(&buf[0], 1, buf.size(), f) == buf.size() && std::fgetc(f) == EOF &&
-             fnv1a64((const uint8_t*)buf.data(), buf.size()) == hd.checksum;
+             (hd.version == 2 ? checkpoint_checksum(hd, buf)
+                              : fnv1a64((const uint8_t*)buf.data(), buf.size())) == hd.checksum;
     }
     std::fclose(f);
     if (!ok) return fail(nullptr, LGCA_ERR_INVALID_STATE, std::string("not a valid checkpoint: ") + path);
@@ -1348,22 +1405,57 @@ extern "C" lgca_status lgca_load_checkpoint(lgca_lattice** out, const char* path
     lgca_status st = lgca_create(&h, hd.model, hd.dim_x, hd.dim_y, hd.coarse_radius, device);
     if (st) return st;
     const size_t n = (size_t)hd.dim_x * hd.dim_y;
+    Controller c{};  // outlives the stream synchronisation below
     {
         std::lock_guard<std::mutex> lk(h->mu);
         DeviceGuard dg(h->device);
         st = upload_types(h, (const uint8_t*)buf.data(), hipMemcpyHostToDevice);
+        // the snapshot first (version 1 files hold none: the saved state stands in)
+        if (!st && hd.version == 2) {
+            st = upload_state(h, (const uint8_t*)buf.data() + 2 * n, hipMemcpyHostToDevice);
+            if (!st) {
+                const hipError_t e =
+                    hipMemcpyAsync(h->snap, h->state[h->cur], h->state_words * 4, hipMemcpyDeviceToDevice, h->stream);
+                if (e != hipSuccess) st = hip_fail(h, e, "hipMemcpyAsync(snapshot)");
+            }
+        }
         if (!st) st = upload_state(h, (const uint8_t*)buf.data() + n, hipMemcpyHostToDevice);
+        if (!st && hd.version == 1) {
+            const hipError_t e =
+                hipMemcpyAsync(h->snap, h->state[h->cur], h->state_words * 4, hipMemcpyDeviceToDevice, h->stream);
+            if (e != hipSuccess) st = hip_fail(h, e, "hipMemcpyAsync(snapshot)");
+        }
         if (!st) {
             h->seed = hd.seed;
             h->step = hd.step;
             h->force_calls = hd.force_calls;
+            // the normalisation of lgca_set_forcing: a window clipped to the
+            // lattice, and no forcing where it is empty or p is 0
             h->force_dir = hd.force_dir;
             h->force_thr = hd.force_thr;
             h->fx0 = hd.fx0;
-            h->fx1 = hd.fx1;
+            h->fx1 = std::min(hd.fx1, hd.dim_x);
             h->fy0 = hd.fy0;
-            h->fy1 = hd.fy1;
+            h->fy1 = std::min(hd.fy1, hd.dim_y);
+            if (h->fx0 >= h->fx1 || h->fy0 >= h->fy1 || h->force_thr == 0) h->force_dir = LGCA_FORCE_NONE;
             h->dec_count = 0;
+        }
+        if (!st && has_ctrl) {
+            c.forcing = hd.ctrl_forcing;
+            c.amount = hd.ctrl_amount;
+            c.reverted = hd.ctrl_reverted;
+            c.calls = (uint32_t)hd.force_calls;
+            c.v[0] = hd.ctrl_v[0];
+            c.v[1] = hd.ctrl_v[1];
+            hipError_t e = dalloc(&h->ctrl, sizeof(Controller));
+            if (e == hipSuccess) e = hipMemcpyAsync(h->ctrl, &c, sizeof c, hipMemcpyHostToDevice, h->stream);
+            if (e != hipSuccess) st = hip_fail(h, e, "controller state");
+            h->ctrl_dir = hd.ctrl_dir;
+            h->ctrl_u = hd.ctrl_u;
+            h->ctrl_eq = hd.ctrl_eq;
+            h->calls_on_device = false;
+        }
+        if (!st) {
             const hipError_t e = hipStreamSynchronize(h->stream);
             if (e != hipSuccess) st = hip_fail(h, e, "hipStreamSynchronize");
         }
@@ -1385,6 +1477,10 @@ extern "C" lgca_status lgca_controller_set(lgca_lattice* h, int dir, float u, in
     if (!h) return LGCA_ERR_INVALID_ARGUMENT;
     if (dir != LGCA_FORCE_X && dir != LGCA_FORCE_Y)
         return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_controller_set: dir must be X or Y");
+    // the viewers' amounts come from get_initial_forcing / get_equilibrium_forcing
+    // (size_t, src/lattice.cpp:445-461) and pass through an int parameter
+    if (forcing < 0 || forcing > INT32_MAX || equilibrium_forcing < 0 || equilibrium_forcing > INT32_MAX)
+        return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_controller_set: amounts must lie in [0, 2^31)");
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
     lgca_status st = sync_force_calls(h);

@@ -490,19 +490,23 @@ void oracle_run(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* types,
 
 /* Count-mode body force (src/omp_lattice.cpp:254-346): draw candidate cells
  * until `forcing` particles are reverted or 2N candidates are spent; the loop
- * runs at least once (do-while).  Candidate i of call c is
+ * runs at least once (do-while).  As in the reference, the count of reverted
+ * particles is an unsigned int compared with the int amount (:263, :345), so
+ * the amount is taken modulo 2^32: a negative one never stops the loop before
+ * the 2N candidates.  Candidate i of call c is
  *   cell = (philox({i_lo, i_hi, c, 3})[1] << 32 | [0]) mod N
  * in place of rand() % N. */
 int64_t oracle_body_force_count(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* types,
                                 uint8_t* state, uint64_t seed, uint64_t call_index,
-                                int force_dir, int64_t forcing)
+                                int force_dir, int32_t forcing)
 {
     const uint64_t n = (uint64_t)dim_x * dim_y;
     const uint64_t it_max = 2 * n;
     uint32_t key[2];
     seed_key(seed, key);
     uint64_t it = 0;
-    int64_t reverted = 0;
+    const uint32_t target = (uint32_t)forcing;
+    uint32_t reverted = 0;
     do {
         uint32_t out[4];
         const uint32_t ctr[4] = { (uint32_t)it, (uint32_t)(it >> 32), (uint32_t)call_index, 3u };
@@ -515,8 +519,8 @@ int64_t oracle_body_force_count(int model, uint32_t dim_x, uint32_t dim_y, const
             reverted += popc8((uint8_t)(s ^ o)) / 2;
             state[cell] = o;
         }
-    } while (reverted < forcing && it < it_max);
-    return reverted;
+    } while (reverted < target && it < it_max);
+    return (int64_t)reverted;
 }
 
 /* ------------------------------------------------------------ post-process */

@@ -79,7 +79,7 @@ void     oracle_run(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* ty
                     uint32_t fy0, uint32_t fy1, int nthreads);
 int64_t  oracle_body_force_count(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* types,
                                  uint8_t* state, uint64_t seed, uint64_t call_index,
-                                 int force_dir, int64_t forcing);
+                                 int force_dir, int32_t forcing);
 
 void     oracle_cell_post_process(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* state,
                                   float* cell_density, float* cell_momentum);

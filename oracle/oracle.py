@@ -63,7 +63,7 @@ def lib() -> C.CDLL:
                                   i32, u64, u32, u32, u32, u32]
         L.oracle_run.argtypes = [i32, u32, u32, _u8p_or_null, _u8p, u64, u64, u64,
                                  i32, u64, u32, u32, u32, u32, i32]
-        L.oracle_body_force_count.argtypes = [i32, u32, u32, _u8p_or_null, _u8p, u64, u64, i32, i64]
+        L.oracle_body_force_count.argtypes = [i32, u32, u32, _u8p_or_null, _u8p, u64, u64, i32, i32]
         L.oracle_body_force_count.restype = i64
         L.oracle_cell_post_process.argtypes = [i32, u32, u32, _u8p, _f32p, _f32p]
         L.oracle_mean_post_process.argtypes = [u32, u32, u32, _f32p, _f32p, _f32p, _f32p]

@@ -332,3 +332,66 @@ def test_invalid_job_descriptions():
             b.result(jid)
         with pytest.raises(LgcaError):
             b.release(jid)
+
+
+def test_serving_stream_matches_single_lattice_path():
+    """The serving workload as a Poisson stream into one batch with the serving
+    cell cap (2^28 active cells): 48 jobs uniform over the four models,
+    256x256..4096x1024, periodic / x-forced pipe (p 1e-4..5e-3) / cylinder of
+    random radius and position, steps cut to 50..400.  Jobs arrive while
+    others run, are admitted as cells free up and share launches with other
+    models.  Every job's state bytes and coarse fields equal its run alone on
+    the single-lattice path; the four smallest also equal the oracle's."""
+    import time as _time
+
+    from lgca_amd import Batch
+    from lgca_amd.workloads import poisson_arrivals, serving_jobs
+    jobs = serving_jobs(48, seed=2026, steps_range=(50, 401))
+    assert {int(j.model) for j in jobs} == {0, 1, 2, 3}
+    assert {j.geometry for j in jobs} == {"periodic", "pipe", "cylinder"}
+    arr = poisson_arrivals(len(jobs), 400.0, seed=3)
+    say(f"[serving] {len(jobs)} jobs over {arr[-1]:.3f} s")
+    with Batch(hbm_budget=200 * 10 ** 9) as b:
+        b.set_tuning(0, 0, 1 << 28)
+        ids = []
+        t0 = _time.monotonic()
+        for i, j in enumerate(jobs):
+            delay = t0 + arr[i] - _time.monotonic()
+            if delay > 0:
+                _time.sleep(delay)
+            ids.append(b.submit(j))
+        b.wait(timeout=300)
+        got = [b.result(i) for i in ids]
+        stats = b.stats()
+    say(f"[serving] stats {stats}")
+    assert stats["jobs_done"] == len(jobs)
+    for i, job in enumerate(jobs):
+        name = f"job {i} m{int(job.model)} {job.dim_x}x{job.dim_y} {job.geometry} {job.steps} steps"
+        check_job(name, got[i], single_lattice_job(job))
+    for i in sorted(range(len(jobs)), key=lambda i: jobs[i].dim_x * jobs[i].dim_y)[:4]:
+        check_job(f"job {i} vs oracle", got[i], oracle_job(jobs[i]))
+
+
+def test_new_job_in_a_freed_slot_gets_its_own_work_items():
+    """A job that runs alone, finishes and frees its slot, then a job of
+    another size admitted into that slot with the same steps per pass and
+    tile height: its work items must be rebuilt for its own geometry (a
+    round signature of slots and steps per pass alone once reused the first
+    job's items)."""
+    from lgca_amd import Batch, Job
+    first = Job(2, 640, 128, 40, seed=31, fill=0.3, coarse_radius=4)
+    later = [Job(0, 1280, 256, 36, seed=32, fill=0.2, geometry="pipe", coarse_radius=4),
+             Job(1, 2496, 192, 44, seed=33, fill=0.25, coarse_radius=4),
+             Job(3, 320, 64, 48, seed=34, fill=0.3, geometry="cylinder", cyl_x=90, cyl_y=30,
+                 cyl_diameter=21.0, force_dir="x", force_p=0.01, coarse_radius=4)]
+    with Batch(hbm_budget=8 * 10 ** 9) as b:
+        b.set_tuning(4, 2048, 0)
+        ids = [b.submit(first)]
+        b.wait(ids[0], timeout=60)
+        for j in later:
+            ids.append(b.submit(j))
+            b.wait(ids[-1], timeout=60)  # each runs alone, in the slot the last one freed
+        got = [b.result(i) for i in ids]
+    for i, job in enumerate([first] + later):
+        say(f"[slot reuse] job {i} m{int(job.model)} {job.dim_x}x{job.dim_y}")
+        check_job(f"job {i}", got[i], oracle_job(job))

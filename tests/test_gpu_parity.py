@@ -287,7 +287,7 @@ def test_init_random_matches_oracle(Lattice, model, p):
 
 
 @pytest.mark.parametrize("model,direction", [(0, "x"), (0, "y"), (1, "y"), (2, "x"), (3, "y")])
-@pytest.mark.parametrize("forcing", [0, 1, 37, 10**9])
+@pytest.mark.parametrize("forcing", [0, 1, 37, 10**9, -1, -(2**31)])
 def test_count_mode_force(Lattice, model, direction, forcing):
     dx, dy = 48, 16
     rng = np.random.default_rng(forcing % 1000 + model)
@@ -327,6 +327,10 @@ def test_invalid_inputs_are_rejected(Lattice):
     t[3] = 5
     with pytest.raises(LgcaError):
         lat.set_cell_types(t)
+    # apply_body_force takes an int: amounts beyond its range are refused
+    for amount in (2 ** 31, -(2 ** 31) - 1):
+        with pytest.raises(LgcaError):
+            lat.apply_body_force_count("x", amount)
     lat.close()
 
 

@@ -414,6 +414,92 @@ def test_count_mode_force(orc):
     assert orc.body_force_count(1, dx, dy, None, a, 1, 0, "y", 1) == 2
 
 
+def _candidates(orc, seed, call, n, stop):
+    """Distinct cells in the order the count-mode candidates first reach them,
+    from the stated stream: candidate i of call c is cell
+    (philox({i_lo, i_hi, c, 3})[1] << 32 | [0]) mod n; `stop(distinct, i)`
+    ends the walk after candidate i."""
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    seen, order, i = set(), [], 0
+    while True:
+        o = orc.philox([i & 0xFFFFFFFF, i >> 32, call, 3], key)
+        cell = ((o[1] << 32) | o[0]) % n
+        i += 1
+        if cell not in seen:
+            seen.add(cell)
+            order.append(cell)
+        if stop(len(order), i):
+            return order
+
+
+def test_count_mode_candidates_are_the_stated_philox_outputs(orc):
+    """Every cell holds one particle the x force can revert, so each first
+    visit reverts exactly one and a revisit none: the cells that flip are the
+    first `forcing` distinct candidates of the stated stream, and a negative
+    amount (taken modulo 2^32, as the reference's unsigned comparison takes
+    it, src/omp_lattice.cpp:263,345) visits all 2N candidates."""
+    dx, dy = 37, 11
+    n = dx * dy
+    seed = 0x1122334455667788
+    for call in (0, 1, 9):
+        for forcing in (1, 5, 60, 400, -1, -(2 ** 31)):
+            st = np.full(n, 1 << 3, dtype=np.uint8)
+            got = orc.body_force_count(2, dx, dy, None, st, seed, call, "x", forcing)
+            if forcing > 0:
+                want = _candidates(orc, seed, call, n,
+                                   lambda k, i: k >= forcing or i >= 2 * n)
+            else:
+                want = _candidates(orc, seed, call, n, lambda k, i: i >= 2 * n)
+            assert got == len(want), (call, forcing)
+            assert sorted(np.flatnonzero(st == 1).tolist()) == sorted(want)
+    # 400 exceeds the distinct cells 2N candidates reach: the loop ends at 2N
+    st = np.full(n, 1 << 3, dtype=np.uint8)
+    assert orc.body_force_count(2, dx, dy, None, st, seed, 0, "x", 400) < 400
+
+
+def test_mean_velocity_closed_form(orc):
+    """get_mean_velocity (src/omp_lattice.cpp:508-557) is the mean over FLUID
+    cells of momentum / density.  Every FLUID cell of the lattice holds the
+    same particles, or half of them hold none: the mean is the one cell's
+    velocity, or half of it; solid cells with particles do not count."""
+    dx, dy = 24, 10
+    types = np.zeros(dx * dy, dtype=np.uint8)
+    types[:dx] = 1                 # a wall row, occupied below
+    types[5 * dx + 3] = 2
+    for model, s, vel in [(0, 0b0001, (1.0, 0.0)), (0, 0b0011, (0.5, 0.5)),
+                          (2, 0b0000011, (0.75, 0.4330127)),   # dirs 0, 1: (1.5, sin60) / 2
+                          (2, 0b1000100, (-0.25, 0.4330127)),   # dir 2 + rest: (-0.5, sin60) / 2
+                          (1, 0b010010, (0.0, 0.0))]:           # dirs 1, 4 cancel
+        st = np.full(dx * dy, s, dtype=np.uint8)
+        cd, cm = orc.cell_post_process(model, dx, dy, st)
+        got = orc.mean_velocity(dx, dy, types, cd, cm)
+        np.testing.assert_allclose(got, vel, rtol=2e-7, atol=1e-12)
+        half = st.copy()
+        half[1::2] = 0             # every other cell empty (each row has even width)
+        cd, cm = orc.cell_post_process(model, dx, dy, half)
+        got = orc.mean_velocity(dx, dy, types, cd, cm)
+        fluid = int((types == 0).sum())
+        occupied = int(((types == 0) & (half != 0)).sum())
+        np.testing.assert_allclose(got, np.array(vel) * occupied / fluid, rtol=2e-7, atol=1e-12)
+
+
+def test_coarse_velocity_closed_form(orc):
+    """Coarse velocity = mean momentum / mean density per coarse cell, 0 where
+    the block is empty: a uniform one-direction state gives the direction's
+    velocity exactly in x (multiples of 1/2 add exactly in FP32) and to FP32
+    rounding in y; an empty block gives 0."""
+    dx, dy, r = 32, 16, 4
+    for model, s, vel in [(0, 0b0010, (0.0, 1.0)), (0, 0b0101, (0.0, 0.0)),
+                          (1, 0b000001, (1.0, 0.0)), (2, 0b0100000, (0.5, -0.8660254))]:
+        st = np.full((dy, dx), s, dtype=np.uint8)
+        st[:, : r + 1] = 0         # the window's r + 1 columns of coarse column 0
+        _, _, md, mm = orc.post_process(model, dx, dy, r, st.reshape(-1))
+        v = orc.coarse_velocity(md, mm).reshape(dy // (2 * r), dx // (2 * r), 2)
+        np.testing.assert_array_equal(v[:, 0], 0.0)
+        np.testing.assert_array_equal(v[:, 1:, 0], np.float32(vel[0]))
+        np.testing.assert_allclose(v[:, 1:, 1], vel[1], rtol=1e-6, atol=0)
+
+
 # ----------------------------------------------------------- post-process
 
 def _window_weights(dx, dy, r, c):
