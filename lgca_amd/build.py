@@ -18,9 +18,10 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblgca_hip.so")
 
 SOURCES = ["lgca_kernels.hip", "lgca_tile.hip", "lgca_capi.hip", "lgca_batch.hip"]
-# lgca_tile_inst.hip is compiled once per (model kind, steps per pass) pair,
-# which lets the template-heavy tiled kernels compile in parallel
-TILE_INST = [(kind, k) for k in (8, 4, 2, 1) for kind in (0, 1, 2)]
+# lgca_tile_inst.hip is compiled once per (model kind, steps per pass) pair of
+# the single-lattice kernels and once per steps per pass of the batch kernel
+# (kind -1), which lets the template-heavy tiled kernels compile in parallel
+TILE_INST = [(kind, k) for k in (8, 4, 2, 1) for kind in (-1, 0, 1, 2)]
 ARCH = os.environ.get("LGCA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -54,7 +55,7 @@ def _compile(job) -> str:
 
 def _jobs():
     jobs = [(f"lgca_tile_inst.hip", [f"-DLGCA_INST_KIND={kind}", f"-DLGCA_INST_K={k}"],
-             f"lgca_tile_m{kind}_k{k}") for kind, k in TILE_INST]
+             f"lgca_tile_{'batch' if kind < 0 else f'm{kind}'}_k{k}") for kind, k in TILE_INST]
     jobs += [(src, [], os.path.splitext(src)[0]) for src in SOURCES]
     return jobs
 

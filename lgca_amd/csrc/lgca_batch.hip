@@ -187,7 +187,6 @@ constexpr size_t kAlign = 256;
 constexpr size_t kStagingHalf = 32u << 20;  // pinned upload staging, two halves
 constexpr uint32_t kMinTileRows = 8, kMaxTileRows = 512;
 constexpr int kTuneRounds = 3;            // rounds timed per candidate
-constexpr uint32_t kOverlapItems = 512;   // classes this large run side by side
 
 size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -366,7 +365,7 @@ struct Arena {
 };
 
 struct ClassRange {
-    int kind, k;
+    int k;
     bool ragged;
     uint32_t offset, count;
     uint32_t heavy;  // leading items that can take the general walk
@@ -385,10 +384,6 @@ struct lgca_batch {
     hipStream_t stream = nullptr;       // worker: set-up and rounds
     hipStream_t post_stream = nullptr;  // worker: post-processing of finished jobs
     hipStream_t copy_stream = nullptr;  // lgca_batch_result
-    // A round's class launches run on `stream` and these two side streams at
-    // once (they advance disjoint jobs), forked and joined with events.
-    hipStream_t class_stream[2] = {nullptr, nullptr};
-    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
     std::thread worker;
 
     std::mutex mu;
@@ -856,7 +851,8 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     }
     if (b->sig == b->last_sig) return;
     b->last_sig = b->sig;
-    // classes in a fixed order: kind, k, ragged (segment and strip jobs share)
+    // classes in a fixed order: k, ragged (every model, segment and strip jobs
+    // share a launch)
     std::map<uint64_t, std::vector<WorkItem>> cls, light;  // light: appended after
     for (size_t i = 0; i < n; ++i) {
         const Job* j = b->active[i];
@@ -866,7 +862,7 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
         const uint32_t halo = seg ? 0u : tile_halo(g, k);
         const uint32_t jth = std::min<uint32_t>(th, g.dim_y + (g.dim_y & 1u));
         const uint32_t tiles = (g.dim_y + jth - 1) / jth;
-        const uint64_t key = ((uint64_t)j->p.kind << 16) | ((uint64_t)k << 8) | (g.nb != 32 ? 1u : 0u);
+        const uint64_t key = ((uint64_t)k << 8) | (g.nb != 32 ? 1u : 0u);
         std::vector<WorkItem>& vh = cls[key];
         std::vector<WorkItem>& vl = light[key];
         if (seg) {
@@ -904,7 +900,6 @@ void build_round(lgca_batch* b, const std::vector<int>& ks, uint32_t th)
     b->classes.clear();
     for (auto& kv : cls) {
         ClassRange c;
-        c.kind = (int)(kv.first >> 16);
         c.k = (int)((kv.first >> 8) & 0xFF);
         c.ragged = (kv.first & 1u) != 0;
         c.offset = (uint32_t)all.size();
@@ -1056,42 +1051,15 @@ void run_round(lgca_batch* b)
 
     if (b->tuning && b->cand_rounds == 0)
         hipchk(hipEventRecord(b->tune_ev[2 * b->cand_i], b->stream), "hipEventRecord");
-    // Classes hold disjoint jobs, so a round's class launches may overlap: a
-    // mix of models gives several launches that each fill only part of the
-    // GPU.  The second and third class of at least kOverlapItems waves run on
-    // the two side streams, forked after the round's uploads and draws and
-    // joined before anything later on the round's stream; the others (the
-    // small classes of jobs at their last steps among them, where the fork
-    // costs more than it saves) stay on the round's stream.
+    // One launch per (k, ragged) class: the jobs of every model share it (each
+    // wave takes its job's model body), so a round is usually one launch, and
+    // only jobs at their last steps add a short one.
     uint64_t nl = 0;
-    const size_t nc = b->classes.size();
-    std::vector<int> lane(nc, -1);  // side stream of each class, -1: the round's
-    int nbig = 0;
-    for (size_t i = 0; i < nc; ++i) {
-        if (b->classes[i].count < kOverlapItems) continue;
-        if (nbig == 1 || nbig == 2) lane[i] = nbig - 1;  // the first stays on the round's stream
-        ++nbig;
-    }
-    bool side_used[2] = {false, false};
-    if (nbig > 1) hipchk(hipEventRecord(b->fork_ev, b->stream), "hipEventRecord(fork)");
-    for (size_t i = 0; i < nc; ++i) {
-        const ClassRange& c = b->classes[i];
-        hipStream_t s = b->stream;
-        if (lane[i] >= 0) {
-            const size_t j = (size_t)lane[i];
-            s = b->class_stream[j];
-            if (!side_used[j]) hipchk(hipStreamWaitEvent(s, b->fork_ev, 0), "hipStreamWaitEvent(fork)");
-            side_used[j] = true;
-        }
-        hipchk(launch_tile_batch(c.kind, c.k, c.ragged, b->d_jobs, b->d_state,
-                                 b->d_items[b->items_buf] + c.offset, c.count, c.heavy, s),
+    for (const ClassRange& c : b->classes) {
+        hipchk(launch_tile_batch(c.k, c.ragged, b->d_jobs, b->d_state, b->d_items[b->items_buf] + c.offset,
+                                 c.count, c.heavy, b->stream),
                "tile_batch_kernel");
         ++nl;
-    }
-    for (size_t j = 0; j < 2; ++j) {
-        if (!side_used[j]) continue;
-        hipchk(hipEventRecord(b->join_ev[j], b->class_stream[j]), "hipEventRecord(join)");
-        hipchk(hipStreamWaitEvent(b->stream, b->join_ev[j], 0), "hipStreamWaitEvent(join)");
     }
     if (b->tuning) {
         b->tune_su[b->cand_i] += su;
@@ -1252,11 +1220,6 @@ extern "C" lgca_status lgca_batch_create(lgca_batch** out, int device, uint64_t 
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
     if ((e = hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
     if ((e = hipStreamCreateWithFlags(&b->post_stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
-    for (auto& cs : b->class_stream)
-        if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return fail(e, "hipStreamCreate");
-    if ((e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "hipEventCreate");
-    for (auto& ev : b->join_ev)
-        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "hipEventCreate");
     if ((e = hipHostMalloc((void**)&b->staging, 2 * kStagingHalf, hipHostMallocDefault)) != hipSuccess)
         return fail(e, "hipHostMalloc(staging)");
     for (auto& ev : b->staging_ev)
@@ -1288,8 +1251,6 @@ extern "C" lgca_status lgca_batch_destroy(lgca_batch* b)
     if (b->worker.joinable()) b->worker.join();
     DevGuard dg(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    for (hipStream_t cs : b->class_stream)
-        if (cs) (void)hipStreamSynchronize(cs);
     if (b->post_stream) (void)hipStreamSynchronize(b->post_stream);
     if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
     for (auto& kv : b->jobs) {
@@ -1301,9 +1262,6 @@ extern "C" lgca_status lgca_batch_destroy(lgca_batch* b)
     for (hipEvent_t e : b->tune_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->staging_ev)
         if (e) (void)hipEventDestroy(e);
-    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
-    for (hipEvent_t e : b->join_ev)
-        if (e) (void)hipEventDestroy(e);
     b->arena.release_all();
     void* dptrs[] = {b->d_jobs, b->d_state, b->d_items[0], b->d_items[1], b->d_draw, b->d_err, b->d_setup};
     for (void* p : dptrs)
@@ -1314,8 +1272,6 @@ extern "C" lgca_status lgca_batch_destroy(lgca_batch* b)
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
     if (b->post_stream) (void)hipStreamDestroy(b->post_stream);
-    for (hipStream_t cs : b->class_stream)
-        if (cs) (void)hipStreamDestroy(cs);
     delete b;
     return LGCA_OK;
 }

@@ -97,20 +97,21 @@ struct WorkItem {
     uint32_t shape;        // rows per tile | halo << 16 | segment mode << 31
     uint32_t job_items;    // items of this job in the round
 };
-// One class launch of a round: every item (all of one model, K and raggedness)
-// advances its job K steps reading its BatchState.  The wave that finishes a
-// job's last item of the round moves the job K steps on and swaps its buffers,
-// for the next round's launches to read.
+// One class launch of a round: every item (all of one K and raggedness, any
+// models) advances its job K steps reading its BatchState.  The wave that
+// finishes a job's last item of the round moves the job K steps on and swaps
+// its buffers, for the next round's launches to read.
 // n_heavy: the class's leading items that can take the general walk.
-hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
+hipError_t launch_tile_batch(int k, bool ragged, const BatchJob* jobs, BatchState* state,
                              const WorkItem* items, uint32_t n_items, uint32_t n_heavy, hipStream_t s);
 
-// Launchers of one (model, K) pair, instantiated by lgca_tile_inst.hip; grid
-// and shapes come checked from launch_tile / launch_tile_batch.
+// Launchers of one (model, K) pair and of one K's batch kernel, instantiated
+// by lgca_tile_inst.hip; grid and shapes come checked from launch_tile /
+// launch_tile_batch.
 template <int KIND, int K>
 hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblocks, bool ragged,
                           bool seg, hipStream_t s);
-template <int KIND, int K>
+template <int K>
 hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
                                 uint32_t n_items, uint32_t nblocks, uint32_t heavy_blocks, bool ragged,
                                 hipStream_t s);

@@ -1,6 +1,6 @@
 // lgca_tile.hip -- host side of the temporally blocked step: launch shapes,
-// grids and the strip-mode block order, dispatched to the per-(model, K)
-// launchers that lgca_tile_inst.hip instantiates from lgca_tile.hpp.
+// grids and the strip-mode block order, dispatched to the per-(model, K) and
+// per-K batch launchers that lgca_tile_inst.hip instantiates from lgca_tile.hpp.
 #include "lgca_dev.hpp"
 #include "lgca_kernels.hpp"
 
@@ -8,18 +8,18 @@
 
 namespace lgca {
 
-hipError_t launch_tile_batch(int kind, int k, bool ragged, const BatchJob* jobs, BatchState* state,
+hipError_t launch_tile_batch(int k, bool ragged, const BatchJob* jobs, BatchState* state,
                              const WorkItem* items, uint32_t n_items, uint32_t n_heavy, hipStream_t s)
 {
     if (!n_items) return hipSuccess;
     const uint32_t nblk = (n_items + 3) / 4, hblk = std::min<uint32_t>((n_heavy + 3) / 4, nblk);
-#define LGCA_BATCH_CASE(KD, KK) \
-    if (kind == KD && k == KK) return launch_tile_batch_kk<KD, KK>(jobs, state, items, n_items, nblk, hblk, ragged, s);
-    LGCA_BATCH_CASE(kHPP, 1) LGCA_BATCH_CASE(kHPP, 2) LGCA_BATCH_CASE(kHPP, 4) LGCA_BATCH_CASE(kHPP, 8)
-    LGCA_BATCH_CASE(kFHP1, 1) LGCA_BATCH_CASE(kFHP1, 2) LGCA_BATCH_CASE(kFHP1, 4) LGCA_BATCH_CASE(kFHP1, 8)
-    LGCA_BATCH_CASE(kFHP2, 1) LGCA_BATCH_CASE(kFHP2, 2) LGCA_BATCH_CASE(kFHP2, 4) LGCA_BATCH_CASE(kFHP2, 8)
-#undef LGCA_BATCH_CASE
-    return hipErrorInvalidValue;
+    switch (k) {
+    case 1: return launch_tile_batch_kk<1>(jobs, state, items, n_items, nblk, hblk, ragged, s);
+    case 2: return launch_tile_batch_kk<2>(jobs, state, items, n_items, nblk, hblk, ragged, s);
+    case 4: return launch_tile_batch_kk<4>(jobs, state, items, n_items, nblk, hblk, ragged, s);
+    case 8: return launch_tile_batch_kk<8>(jobs, state, items, n_items, nblk, hblk, ragged, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 uint32_t tile_halo(const Geometry& g, int k)

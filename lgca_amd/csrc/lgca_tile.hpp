@@ -26,8 +26,9 @@
 // the group's later levels take their word from a delay line, so a pass makes
 // one call per word and row per group it touches, whatever its first step.
 //
-// The kernels are templates; lgca_tile_inst.hip instantiates one (model, K)
-// pair per translation unit, and lgca_tile.hip dispatches to them.
+// The kernels are templates; lgca_tile_inst.hip instantiates the single-
+// lattice kernels of one (model, K) pair, or the batch kernel of one K, per
+// translation unit, and lgca_tile.hip dispatches to them.
 #pragma once
 
 #include "lgca_dev.hpp"
@@ -597,10 +598,13 @@ void tile_kernel(TileArgs a, TileOrder o)
 // Batched form: every wave takes one work item (job, strip, tile, shape) and
 // runs that job's lattice.  A job's step, current buffer and force-window
 // origin live in device memory (the job's last wave of a round moves them on),
-// so queued rounds need no host-to-device traffic and no launch between them.  One launch covers the
-// items of one (model, K, ragged) class; the top bit of an item's shape picks
-// segment mode for its job, so strip and segment jobs share the launch.
-template <int KIND, int K, bool RAGGED>
+// so queued rounds need no host-to-device traffic and no launch between them.
+// One launch covers all items of one (K, ragged) class, whatever their jobs'
+// models: each wave takes its job's model body (the job's Q, wave-uniform), so
+// a round of a mixed batch is one launch rather than one per model.  The top
+// bit of an item's shape picks segment mode for its job, so strip and segment
+// jobs share the launch too.
+template <int K, bool RAGGED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
                        const WorkItem* __restrict__ items, uint32_t n_items, uint32_t heavy_blocks)
@@ -628,8 +632,18 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
     // batch rounds start passes of K >= 4 steps on multiples of 4 (O = 0) and
     // 2-step passes on even steps (picked at run time)
     constexpr int O = K <= 2 ? kRuntimeOffset : 0;
-    if (it.shape >> 31) tile_body<KIND, K, O, RAGGED, true>(a, it.strip, it.tile);
-    else tile_body<KIND, K, O, RAGGED, false>(a, it.strip, it.tile);
+    const bool seg = (it.shape >> 31) != 0;
+    const uint32_t Q = __builtin_amdgcn_readfirstlane(a.g.Q);
+    if (Q == 4) {
+        if (seg) tile_body<kHPP, K, O, RAGGED, true>(a, it.strip, it.tile);
+        else tile_body<kHPP, K, O, RAGGED, false>(a, it.strip, it.tile);
+    } else if (Q == 6) {
+        if (seg) tile_body<kFHP1, K, O, RAGGED, true>(a, it.strip, it.tile);
+        else tile_body<kFHP1, K, O, RAGGED, false>(a, it.strip, it.tile);
+    } else {
+        if (seg) tile_body<kFHP2, K, O, RAGGED, true>(a, it.strip, it.tile);
+        else tile_body<kFHP2, K, O, RAGGED, false>(a, it.strip, it.tile);
+    }
     // The job's last wave of the round advances it.  Every other wave of the
     // job has read the state and counted itself, so nothing in this launch
     // reads the state after the update, and the next launch on the stream
@@ -676,17 +690,17 @@ hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblock
     return hipErrorInvalidValue;
 }
 
-template <int KIND, int K>
+template <int K>
 hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
                                 uint32_t n_items, uint32_t nblocks, uint32_t heavy_blocks, bool ragged,
                                 hipStream_t s)
 {
     if (ragged)
-        hipLaunchKernelGGL((tile_batch_kernel<KIND, K, true>), dim3(nblocks), dim3(256), 0, s, jobs, state,
-                           items, n_items, heavy_blocks);
+        hipLaunchKernelGGL((tile_batch_kernel<K, true>), dim3(nblocks), dim3(256), 0, s, jobs, state, items,
+                           n_items, heavy_blocks);
     else
-        hipLaunchKernelGGL((tile_batch_kernel<KIND, K, false>), dim3(nblocks), dim3(256), 0, s, jobs, state,
-                           items, n_items, heavy_blocks);
+        hipLaunchKernelGGL((tile_batch_kernel<K, false>), dim3(nblocks), dim3(256), 0, s, jobs, state, items,
+                           n_items, heavy_blocks);
     return hipGetLastError();
 }
 
