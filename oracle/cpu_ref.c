@@ -24,9 +24,13 @@
  *   oracle_mean_post_process  indicator-field probes of the averaging window
  *                             (each cell's membership in each coarse cell) and
  *                             closed-form counts
- *   oracle_body_force_count   hand-worked cases of the stopping rule; the
+ *   oracle_body_force_count   hand-worked cases of the stopping rule, and the
+ *                             stated Philox outputs of the candidate stream; the
  *                             candidate sequence itself is this build's contract
  *                             (the reference draws rand(), which nothing pins)
+ *   oracle_mean_velocity,     closed forms on uniform states
+ *   oracle_coarse_velocity
+ * oracle_fast_run (cpu_fast.c) is pinned by equality with oracle_run.
  */
 #include "cpu_ref.h"
 
@@ -86,7 +90,7 @@ uint32_t oracle_chirality_word(uint64_t seed, uint64_t step, uint32_t y, uint32_
  * C[m] = P(first fire within m + 1 cells) * 2^32 = 2^32 - Q_{m+1}, where
  * Q_k = floor(2^32 (1 - p)^k) is built by exact integer steps so every
  * platform gets the same table. */
-static void force_cdf(uint64_t thr, uint64_t C[32])
+void oracle_force_table(uint64_t thr, uint64_t C[32])
 {
     const uint64_t q = ((uint64_t)1 << 32) - thr;  /* thr >= 1, so q < 2^32 */
     uint64_t Qk = q;                               /* Q_1 */
@@ -123,8 +127,8 @@ static uint32_t force_walk(const uint32_t key[2], const uint64_t C[32], uint32_t
 
 /* Walks the steps of the word's four-step group up to `step`: they share the
  * group's extras in step order. */
-static uint32_t force_word_cdf(const uint32_t key[2], const uint64_t C[32], uint64_t step,
-                               uint32_t y, uint32_t w)
+uint32_t oracle_force_word_with(const uint32_t key[2], const uint64_t C[32], uint64_t step,
+                                uint32_t y, uint32_t w)
 {
     uint32_t prim[4], ext[4] = { 0, 0, 0, 0 };
     const uint32_t group = (uint32_t)step >> 2;
@@ -143,8 +147,8 @@ uint32_t oracle_force_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w,
     uint32_t key[2];
     uint64_t C[32];
     seed_key(seed, key);
-    force_cdf(thr, C);
-    return force_word_cdf(key, C, step, y, w);
+    oracle_force_table(thr, C);
+    return oracle_force_word_with(key, C, step, y, w);
 }
 
 uint32_t oracle_fill_draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t d)
@@ -384,7 +388,7 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
     uint32_t key[2];
     uint64_t fcdf[32];
     seed_key(seed, key);
-    if (force_thr != 0) force_cdf(force_thr, fcdf);
+    if (force_thr != 0) oracle_force_table(force_thr, fcdf);
     for (uint32_t y = y_begin; y < y_end; ++y) {
         uint32_t chir[4] = { 0, 0, 0, 0 };
         uint32_t chir_word = UINT32_MAX;
@@ -429,7 +433,7 @@ static void step_rows(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* 
                 o = lut->lut[p][s];
                 if (row_forced && x >= fx0 && x < fx1) {
                     if ((x >> 5) != fword_w) {
-                        fword = force_word_cdf(key, fcdf, step, y, x >> 5);
+                        fword = oracle_force_word_with(key, fcdf, step, y, x >> 5);
                         fword_w = x >> 5;
                     }
                     if ((fword >> (x & 31)) & 1u) o = apply_force(model, force_dir, o);

@@ -56,6 +56,11 @@ void     oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint
 uint64_t oracle_threshold(double p);
 uint32_t oracle_chirality_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w);
 uint32_t oracle_force_word(uint64_t seed, uint64_t step, uint32_t y, uint32_t w, uint64_t thr);
+/* The same word from a prepared table C[m] = 2^32 - Q_{m+1} (oracle_force_table)
+ * and key {seed_lo, seed_hi}: shared by cpu_ref.c and cpu_fast.c. */
+void     oracle_force_table(uint64_t thr, uint64_t C[32]);
+uint32_t oracle_force_word_with(const uint32_t key[2], const uint64_t C[32], uint64_t step,
+                                uint32_t y, uint32_t w);
 uint32_t oracle_fill_draw(uint64_t seed, uint32_t x, uint32_t y, uint32_t d);
 
 int      oracle_num_dir(int model);
@@ -77,6 +82,14 @@ void     oracle_run(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* ty
                     uint8_t* state, uint64_t seed, uint64_t step0, uint64_t nsteps,
                     int force_dir, uint64_t force_thr, uint32_t fx0, uint32_t fx1,
                     uint32_t fy0, uint32_t fy1, int nthreads);
+/* oracle_run computed bit-sliced, 64 cells per word (cpu_fast.c): same
+ * arguments, same result, many times faster.  Returns 0, or -1 (state
+ * untouched) when its word-level collision disagrees with oracle_collide on
+ * some input, which it checks on every call. */
+int      oracle_fast_run(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* types,
+                         uint8_t* state, uint64_t seed, uint64_t step0, uint64_t nsteps,
+                         int force_dir, uint64_t force_thr, uint32_t fx0, uint32_t fx1,
+                         uint32_t fy0, uint32_t fy1, int nthreads);
 int64_t  oracle_body_force_count(int model, uint32_t dim_x, uint32_t dim_y, const uint8_t* types,
                                  uint8_t* state, uint64_t seed, uint64_t call_index,
                                  int force_dir, int32_t forcing);

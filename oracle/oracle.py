@@ -1,4 +1,5 @@
-"""ctypes front end of the CPU oracle (oracle/cpu_ref.c) -- test infrastructure only.
+"""ctypes front end of the CPU oracle (oracle/cpu_ref.c, oracle/cpu_fast.c) -- test
+infrastructure only.
 
 The oracle restates the reference's OpenMP path (src/omp_lattice.cpp,
 src/lattice.cpp, src/lgca_models.h) on one byte per cell.  Only tests/,
@@ -63,6 +64,9 @@ def lib() -> C.CDLL:
                                   i32, u64, u32, u32, u32, u32]
         L.oracle_run.argtypes = [i32, u32, u32, _u8p_or_null, _u8p, u64, u64, u64,
                                  i32, u64, u32, u32, u32, u32, i32]
+        L.oracle_fast_run.argtypes = [i32, u32, u32, _u8p_or_null, _u8p, u64, u64, u64,
+                                      i32, u64, u32, u32, u32, u32, i32]
+        L.oracle_fast_run.restype = i32
         L.oracle_body_force_count.argtypes = [i32, u32, u32, _u8p_or_null, _u8p, u64, u64, i32, i32]
         L.oracle_body_force_count.restype = i64
         L.oracle_cell_post_process.argtypes = [i32, u32, u32, _u8p, _f32p, _f32p]
@@ -131,6 +135,15 @@ def run(model, dim_x, dim_y, types, state, seed, step0, nsteps, force=None, nthr
     st = np.ascontiguousarray(state).copy()
     lib().oracle_run(model, dim_x, dim_y, _types_ptr(types), st, seed, step0, nsteps,
                      *force_args(force), nthreads)
+    return st
+
+
+def fast_run(model, dim_x, dim_y, types, state, seed, step0, nsteps, force=None, nthreads=0):
+    """run() computed bit-sliced (cpu_fast.c), for long runs; pinned to run()."""
+    st = np.ascontiguousarray(state).copy()
+    if lib().oracle_fast_run(model, dim_x, dim_y, _types_ptr(types), st, seed, step0, nsteps,
+                             *force_args(force), nthreads) != 0:
+        raise RuntimeError("oracle_fast_run: its collision disagrees with oracle_collide")
     return st
 
 

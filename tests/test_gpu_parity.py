@@ -436,3 +436,44 @@ def test_tune_keeps_state_and_step(Lattice):
     lat.tune()
     assert lat.info().tile_rows == 10
     lat.close()
+
+
+@pytest.mark.slow
+@pytest.mark.timeout(1200)
+def test_full_karman_5000_steps_bit_exact(Lattice):
+    """All 5000 steps of the FHP-III Karman workload at its full size
+    (32768x8192, cylinder d = 512 at (4096, 4099), walls, fill 0.25, x force
+    p = 2e-3 on x < 64, seed 11, r = 32), through the tuned default launch
+    shape, against the bit-sliced CPU checker (oracle/cpu_fast.c, itself pinned
+    to oracle/cpu_ref.c by test_oracle.py): final state and coarse fields."""
+    import os
+    dx, dy, steps = 32768, 8192, 5000
+    threads = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    say(f"[full 5000] karman 32768x8192, checker on {threads} threads")
+    lat = Lattice(3, dim_x=dx, dim_y=dy, coarse_graining_radius=32, seed=11)
+    lat.apply_bc_cylinder(4096, 4099, 512.0)
+    lat.init_random(0.25)
+    lat.set_forcing("x", 2e-3, 0, 64)
+    types = lat.m_cell_type_cpu.copy()
+    state = lat.get_state_u8()
+    lat.tune()
+    lat.step(steps)
+    info = lat.info()
+    say(f"[full 5000] GPU done: steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
+    got = lat.get_state_u8()
+    lat.copy_data_to_output_buffer()
+    lat.post_process(cells=False)
+    md_gpu, mm_gpu = lat.mean_density().copy(), lat.mean_momentum().copy()
+    lat.close()
+    want = orc.fast_run(3, dx, dy, types, state, 11, 0, steps, dict(dir="x", p=2e-3, x0=0, x1=64),
+                        nthreads=threads)
+    del state
+    say("[full 5000] checker done")
+    np.testing.assert_array_equal(got, want)
+    del got
+    _, _, md, mm = orc.post_process(3, dx, dy, 32, want)
+    np.testing.assert_array_equal(md_gpu.view(np.uint32), md.view(np.uint32))
+    np.testing.assert_array_equal(mm_gpu.view(np.uint32), mm.view(np.uint32))

@@ -582,3 +582,60 @@ def test_init_random_fraction_and_solids(orc):
     bits = np.unpackbits(st[dx:, None], axis=1, bitorder="little")[:, :7]
     assert abs(bits.mean() - 0.25) < 0.01
     assert (np.unpackbits(st[:, None], axis=1, bitorder="little")[:, 7] == 0).all()
+
+
+# ------------------------------------------------------ bit-sliced checker
+
+def _types_for(rng, kind, dx, dy):
+    t = np.zeros((dy, dx), dtype=np.uint8)
+    if kind == "pipe":
+        t[0, :] = t[-1, :] = 1
+    elif kind == "mixed":
+        t[rng.random((dy, dx)) < 0.07] = 1
+        t[rng.random((dy, dx)) < 0.05] = 2
+        t[:, 0] = 2
+        t[-1, :] = 2
+        t[0, : dx // 2] = 1
+    elif kind == "slipbox":
+        t[0, :] = t[-1, :] = t[:, 0] = t[:, -1] = 2
+    return None if kind == "periodic" else t.reshape(-1)
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3])
+@pytest.mark.parametrize("dims", [(1, 2), (2, 2), (3, 4), (63, 6), (64, 4), (65, 8), (130, 10),
+                                  (200, 12), (257, 2)])
+def test_fast_run_equals_run(orc, model, dims):
+    """oracle_fast_run (64 cells per word) equals oracle_run cell for cell on
+    every model, on widths around the 64-cell word (ragged last words, a
+    lattice one cell wide), with walls, slip and bounce-back cells, x and y
+    forcing on regions that cut words, and runs starting at every step of a
+    four-step chirality group."""
+    dx, dy = dims
+    if model != 0 and dy % 2:
+        dy += 1
+    rng = np.random.default_rng(dx * 7 + dy + model)
+    for kind, force, step0 in [("periodic", None, 0), ("pipe", dict(dir="x", p=0.3), 1),
+                               ("mixed", dict(dir="y", p=0.4, x0=dx // 3, x1=dx - dx // 4,
+                                              y0=1, y1=dy), 2),
+                               ("slipbox", dict(dir="x", p=1.0, x0=1, x1=max(2, dx // 2)), 3)]:
+        types = _types_for(rng, kind, dx, dy)
+        st = rng.integers(0, 1 << Q[model], dx * dy, dtype=np.uint8)
+        want = orc.run(model, dx, dy, types, st, 99 + step0, step0, 13, force, nthreads=4)
+        got = orc.fast_run(model, dx, dy, types, st, 99 + step0, step0, 13, force, nthreads=4)
+        np.testing.assert_array_equal(got, want, err_msg=f"{kind} {force}")
+
+
+def test_fast_run_equals_run_on_a_karman_like_lattice(orc):
+    """A scaled Karman set-up (cylinder, walls, x force on the first 64
+    columns, fill 0.25) over 40 steps from step 5."""
+    dx, dy = 1024, 256
+    t = np.zeros((dy, dx), dtype=np.uint8)
+    t[0, :] = t[-1, :] = 1
+    ys, xs = np.mgrid[0:dy, 0:dx]
+    t[(xs - 128) ** 2 + (ys - 131) ** 2 < 32 ** 2] = 1
+    types = t.reshape(-1)
+    st = orc.init_random(3, dx, dy, types, 11, 0.25)
+    force = dict(dir="x", p=2e-3, x0=0, x1=64)
+    want = orc.run(3, dx, dy, types, st, 11, 5, 40, force, nthreads=8)
+    got = orc.fast_run(3, dx, dy, types, st, 11, 5, 40, force, nthreads=8)
+    np.testing.assert_array_equal(got, want)
