@@ -149,7 +149,13 @@ template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 
 // form S = 64 / W segments of W lanes, each holding a whole row; segment s
 // walks row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when
 // K == 1, whose one level picks its word at run time), or kRuntimeOffset.
-template <int KIND, int K, int O, bool RAGGED, bool SEG>
+//
+// ALT (strip mode): odd row tiles walk up the lattice, even ones down.  Two
+// vertically adjacent tiles then read the K halo rows they share at the same
+// point of their walks (a down tile ends where the up tile below it ends, and
+// starts where the up tile above it starts), so the second read of each halo
+// row finds it in the L2 cache of their XCD instead of HBM.
+template <int KIND, int K, int O, bool RAGGED, bool SEG, bool ALT = false>
 __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile)
 {
     constexpr int Q = num_dir_of_kind(KIND);
@@ -236,6 +242,12 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         }
         return (uint32_t)r;
     };
+    // Walk direction (wave-uniform) and the input row of iteration i: down
+    // from y0 - K, or up from y0 + rows - 1 + K.
+    const bool up = ALT && !SEG && (tile & 1u) != 0;
+    auto iter_row = [&](uint32_t i) -> uint32_t {
+        return up ? wrap((int)(y0 + rows - 1) + K - (int)i) : wrap((int)(y0 + i) - K);
+    };
 
     // Wave geometry for the uniform mask / force tests: does the wave (words
     // [wlo, wlo + 64) modulo W) touch words [lo, hi)?
@@ -260,7 +272,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         bool hit = false;
         const uint32_t i = i0 + lane;
         if (i < n_iter) {
-            const uint32_t r = wrap((int)(y0 + i) - K);
+            const uint32_t r = iter_row(i);
             hit = wave_hits(a.row_range[r], a.row_range[g.dim_y + r]);
         }
         return __builtin_amdgcn_ballot_w64(hit);
@@ -273,7 +285,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     // walks on small lattices feel it).  At K = 4 the registers this holds
     // across the tests push the walks into spilling, so the walk loads it.
     constexpr bool kEarlyRow = K <= 2;
-    const uint32_t row_first = wrap((int)y0 - K);
+    const uint32_t row_first = iter_row(0);
     uint32_t first[7];
 #pragma unroll
     for (int d = 0; d < Q; ++d) first[d] = kEarlyRow ? load_plane(row_first, d) : 0u;
@@ -311,17 +323,20 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
             const uint32_t i = i0 + lane;
             bool f = false;
             if (i < n_iter) {
-                const uint32_t r = wrap((int)(y0 + i) - K);
+                const uint32_t r = iter_row(i);
                 f = r >= a.fy0 && r < a.fy1;
             }
             tile_force = __builtin_amdgcn_ballot_w64(f) != 0;
         }
     }
 
-    auto walk = [&](auto side_tag) {
+    auto walk = [&](auto side_tag, auto up_tag) {
     constexpr bool SIDE = decltype(side_tag)::value;
+    constexpr bool UP = decltype(up_tag)::value;
     const bool t_solids = SIDE && tile_solids, t_force = SIDE && tile_force;
-    // pipeline windows between level b and b+1: rows j-1 (planes 1, 2) and j
+    // pipeline windows between level b and b+1: rows j and the one before it
+    // in walk order (j-1 walking down, j+1 walking up), of which only the
+    // planes pulled across rows are kept (1, 2 down and 4, 5 up; HPP 1 or 3)
     uint32_t winA[K][7];
     uint32_t winB[K][2];
     uint32_t dl[K + 1][4];            // chirality delay lines
@@ -346,7 +361,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     uint32_t row_i = row_first;
     uint32_t lrow[K + 1];
 #pragma unroll
-    for (int s = 1; s <= K; ++s) lrow[s] = wrap((int)row_i - s);
+    for (int s = 1; s <= K; ++s) lrow[s] = wrap(UP ? (int)row_i + s : (int)row_i - s);
 
     uint32_t nxt[7];
     uint32_t nns = 0, nsl = 0;  // masks of the prefetched row
@@ -360,12 +375,14 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     }
 
     // One step of the row walk.  STEADY iterations (i >= 2K) have every level
-    // active.  PAR is i & 1: with even tile starts and an even dim_y (FHP), the
-    // row of level s has parity (K + PAR + s) & 1.
+    // active.  PAR is i & 1: with even tile starts, even tile heights and an
+    // even dim_y (FHP), the row of level s has parity (K + PAR + s) & 1 walking
+    // down and (1 + K + PAR + s) & 1 walking up.
     auto iterate = [&](auto steady_tag, auto par_tag, uint32_t i) {
         constexpr bool STEADY = decltype(steady_tag)::value;
         constexpr int PAR = decltype(par_tag)::value;
-        const uint32_t row_next = row_i + 1 == g.dim_y ? 0u : row_i + 1;
+        const uint32_t row_next = UP ? (row_i == 0 ? g.dim_y - 1u : row_i - 1u)
+                                     : (row_i + 1 == g.dim_y ? 0u : row_i + 1);
         uint32_t cur[7];
 #pragma unroll
         for (int d = 0; d < Q; ++d) cur[d] = nxt[d];
@@ -390,7 +407,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                 // there is no decision
 #pragma unroll
                 for (int s = 1; s <= K; ++s) {
-                    // next iteration, level s works on row_i - (s - 1)
+                    // next iteration, level s works on the row level s - 1 works on now
                     const uint32_t jn = s == 1 ? row_i : lrow[s - 1];
                     const bool lev = STEADY || i + 1 >= (uint32_t)(2 * s);
                     const uint32_t* base = a.fdec + (size_t)(s - 1) * a.fdec_stride;
@@ -417,31 +434,36 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
             const bool have_prev = STEADY || i >= (uint32_t)(2 * (s - 1));
             const uint32_t j = lrow[s];                      // row of level s
             const uint32_t t = a.t0 + (uint32_t)(s - 1);     // its step
-            const bool odd_row = KIND != kHPP && (((K + PAR + s) & 1) != 0);
+            const bool odd_row = KIND != kHPP && (((K + PAR + s + (UP ? 1 : 0)) & 1) != 0);
             if (active) {
-                // pull stream (src/omp_lattice.cpp:141-180)
+                // pull stream (src/omp_lattice.cpp:141-180).  Planes 1 and 2
+                // (HPP: 1) come from row j-1, planes 4 and 5 (HPP: 3) from
+                // row j+1: the kept older row and the newest row, in the
+                // order of the walk's direction.
                 uint32_t in[7];
                 const uint32_t* A = winA[s - 1];
                 const uint32_t* B = winB[s - 1];
                 if (KIND == kHPP) {
-                    in[0] = shift_m1<RAGGED, SEG>(A[0], c);            // from (x-1, j)
-                    in[1] = B[0];                         // from (x, j-1), plane 1 kept in B[0]
+                    in[0] = shift_m1<RAGGED, SEG>(A[0], c);    // from (x-1, j)
+                    in[1] = UP ? cur[1] : B[0];                // from (x, j-1)
                     in[2] = shift_p1<RAGGED, SEG>(A[2], c);    // from (x+1, j)
-                    in[3] = cur[3];                       // from (x, j+1)
+                    in[3] = UP ? B[0] : cur[3];                // from (x, j+1)
                 } else {
+                    const uint32_t r1 = UP ? cur[1] : B[0], r2 = UP ? cur[2] : B[1];  // row j-1
+                    const uint32_t r4 = UP ? B[0] : cur[4], r5 = UP ? B[1] : cur[5];  // row j+1
                     in[0] = shift_m1<RAGGED, SEG>(A[0], c);
                     in[3] = shift_p1<RAGGED, SEG>(A[3], c);
                     if (Q == 7) in[6] = A[6];
                     if (odd_row) {
-                        in[1] = B[0];
-                        in[2] = shift_p1<RAGGED, SEG>(B[1], c);
-                        in[4] = shift_p1<RAGGED, SEG>(cur[4], c);
-                        in[5] = cur[5];
+                        in[1] = r1;
+                        in[2] = shift_p1<RAGGED, SEG>(r2, c);
+                        in[4] = shift_p1<RAGGED, SEG>(r4, c);
+                        in[5] = r5;
                     } else {
-                        in[1] = shift_m1<RAGGED, SEG>(B[0], c);
-                        in[2] = B[1];
-                        in[4] = cur[4];
-                        in[5] = shift_m1<RAGGED, SEG>(cur[5], c);
+                        in[1] = shift_m1<RAGGED, SEG>(r1, c);
+                        in[2] = r2;
+                        in[4] = r4;
+                        in[5] = shift_m1<RAGGED, SEG>(r5, c);
                     }
                 }
                 uint32_t st[7];
@@ -474,10 +496,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
             // slide the window of boundary s-1 (its newest row is cur)
             if (have_prev) {
                 if (KIND == kHPP) {
-                    winB[s - 1][0] = winA[s - 1][1];
+                    winB[s - 1][0] = winA[s - 1][UP ? 3 : 1];
                 } else {
-                    winB[s - 1][0] = winA[s - 1][1];
-                    winB[s - 1][1] = winA[s - 1][2];
+                    winB[s - 1][0] = winA[s - 1][UP ? 4 : 1];
+                    winB[s - 1][1] = winA[s - 1][UP ? 5 : 2];
                 }
 #pragma unroll
                 for (int d = 0; d < Q; ++d) winA[s - 1][d] = cur[d];
@@ -549,8 +571,16 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     // copy of the walk with that code left out (no tests, no branches).  Rows
     // with a short last word keep the one walk: the second copy would push
     // their kernels past the register budget.
-    if (RAGGED || tile_solids || tile_force) walk(std::true_type{});
-    else walk(std::false_type{});
+    const bool side = RAGGED || tile_solids || tile_force;
+    if constexpr (ALT && !SEG) {
+        if (up) {
+            if (side) walk(std::true_type{}, std::true_type{});
+            else walk(std::false_type{}, std::true_type{});
+            return;
+        }
+    }
+    if (side) walk(std::true_type{}, std::false_type{});
+    else walk(std::false_type{}, std::false_type{});
 }
 
 // Strip-mode block order (TileOrder): a block holds four consecutive strips
@@ -592,7 +622,7 @@ void tile_kernel(TileArgs a, TileOrder o)
             if (o.heavy[h] <= bx) ++bx;
     }
     const uint32_t strip = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
-    tile_body<KIND, K, O, RAGGED, false>(a, strip, tile);
+    tile_body<KIND, K, O, RAGGED, false, true>(a, strip, tile);
 }
 
 // Batched form: every wave takes one work item (job, strip, tile, shape) and
