@@ -161,6 +161,15 @@ def time_steps(w, steps: int, warmup: int, tuning=None, tune: bool = True):
     wall = time.perf_counter() - t0
     ms = start.elapsed_ms(end)
     n1 = lat.get_n_particles()
+    # the coarse fields of the final state (snapshot, then the coarse kernel)
+    lat.copy_data_to_output_buffer()
+    lat.post_process_device()  # allocates the coarse arrays; the second call is timed
+    stream.synchronize()
+    start.record(stream)
+    lat.post_process_device()
+    end.record(stream)
+    end.synchronize()
+    coarse_ms = start.elapsed_ms(end)
     info = lat.info()
     lat.close()
     start.close()
@@ -169,7 +178,8 @@ def time_steps(w, steps: int, warmup: int, tuning=None, tune: bool = True):
     sups = n * steps / (ms / 1e3)
     achieved = n * 2 * Q[w["model"]] / 8 * steps / (ms / 1e3) / 1e9
     return dict(ms=ms, wall=wall, sups=sups, achieved=achieved, conserved=n0 == n1,
-                steps_per_pass=int(info.steps_per_pass), tile_rows=int(info.tile_rows))
+                steps_per_pass=int(info.steps_per_pass), tile_rows=int(info.tile_rows),
+                coarse_ms=coarse_ms)
 
 
 def other_configs():
@@ -383,6 +393,7 @@ def main() -> int:
                      "traffic": traffic},
         "particles_conserved": r["conserved"],
         "wall_s": round(r["wall"], 3),
+        "coarse_fields_ms": round(r["coarse_ms"], 3),
     }
     if not args.no_other_configs:
         rec["single_lattice"] = other_configs()
