@@ -136,8 +136,10 @@ __global__ __launch_bounds__(256) void setup_kernel(const JobInstall* __restrict
 }
 
 // Force windows of many jobs: blockIdx.y picks the job, threads stride over
-// its window's (group of four steps, row, word) triples.  The window starts at
-// the job's current device step, which becomes its fdec_step0.
+// its force region's (row, word) pairs and draw every four-step group of the
+// window for theirs.  The window starts at the job's current device step,
+// which becomes its fdec_step0.  (A thread per (group, row, word) triple paid
+// two 32-bit divisions per triple to find its row and word.)
 __global__ __launch_bounds__(256) void batch_draw_kernel(const BatchJob* __restrict__ jobs,
                                                          BatchState* state,
                                                          const uint32_t* __restrict__ slots,
@@ -151,27 +153,20 @@ __global__ __launch_bounds__(256) void batch_draw_kernel(const BatchJob* __restr
     const uint32_t rows = j.a.fy1 - j.a.fy0, fwn = j.a.fwn;
     const uint32_t groups = ((step0 & 3u) + nsteps + 3u) / 4u;
     const uint64_t plane = (uint64_t)rows * fwn;
-    const uint64_t n = groups * plane;
     uint32_t* dec = const_cast<uint32_t*>(j.a.fdec);
-    if (n < ((uint64_t)1 << 32)) {  // 32-bit divisions, the usual case
-        const uint32_t p32 = (uint32_t)plane;
-        for (uint32_t idx = blockIdx.x * 256 + threadIdx.x; idx < (uint32_t)n; idx += gridDim.x * 256) {
-            const uint32_t g = idx / p32, rem = idx - g * p32;
-            const uint32_t yy = rem / fwn, ww = rem - yy * fwn;
-            const uint32_t w = j.a.fw0 + ww;
-            draw_group(dec + rem, plane, nsteps, step0, g, w, j.a.fy0 + yy, j.a.key0, j.a.key1, Qt,
-                       force_columns(w, j.fx0, j.fx1, j.a.g.dim_x));
+    for (uint64_t it = (uint64_t)blockIdx.x * 256 + threadIdx.x; it < plane; it += (uint64_t)gridDim.x * 256) {
+        uint32_t yy, ww;
+        if (plane < ((uint64_t)1 << 32)) {  // 32-bit division, the usual case
+            yy = (uint32_t)it / fwn;
+            ww = (uint32_t)it - yy * fwn;
+        } else {
+            yy = (uint32_t)(it / fwn);
+            ww = (uint32_t)(it - (uint64_t)yy * fwn);
         }
-    } else {
-        for (uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x; idx < n;
-             idx += (uint64_t)gridDim.x * 256) {
-            const uint32_t g = (uint32_t)(idx / plane);
-            const uint64_t rem = idx - g * plane;
-            const uint32_t yy = (uint32_t)(rem / fwn), ww = (uint32_t)(rem - (uint64_t)yy * fwn);
-            const uint32_t w = j.a.fw0 + ww;
-            draw_group(dec + rem, plane, nsteps, step0, g, w, j.a.fy0 + yy, j.a.key0, j.a.key1, Qt,
-                       force_columns(w, j.fx0, j.fx1, j.a.g.dim_x));
-        }
+        const uint32_t w = j.a.fw0 + ww;
+        const uint32_t cols = force_columns(w, j.fx0, j.fx1, j.a.g.dim_x);
+        for (uint32_t g = 0; g < groups; ++g)
+            draw_group(dec + it, plane, nsteps, step0, g, w, j.a.fy0 + yy, j.a.key0, j.a.key1, Qt, cols);
     }
     // fdec_step0 is not the field the other blocks read
     if (blockIdx.x == 0 && threadIdx.x == 0) state[slot].fdec_step0 = step0;
@@ -939,7 +934,7 @@ void draw_windows(lgca_batch* b, const std::vector<int>& ks)
     uint64_t most = 0;
     for (uint32_t s : slots) {
         const Job* j = b->slot_job[s];
-        most = std::max<uint64_t>(most, (uint64_t)(kWindow / 4 + 1) * (j->p.fy1 - j->p.fy0) * j->p.fwn);
+        most = std::max<uint64_t>(most, (uint64_t)(j->p.fy1 - j->p.fy0) * j->p.fwn);
     }
     const uint32_t bx = (uint32_t)std::min<uint64_t>((most + 255) / 256, 128);
     // blockIdx.y covers at most 65535 jobs per launch

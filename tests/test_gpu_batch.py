@@ -14,6 +14,7 @@ import time
 import numpy as np
 import pytest
 
+from lgca_amd.lattice import model_id
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -51,7 +52,7 @@ def force_dict(job):
 
 
 def oracle_job(job):
-    m = int(job.model)
+    m = model_id(job.model)
     types = job_types(job)
     if job.state is not None:
         st = np.ascontiguousarray(job.state, dtype=np.uint8).reshape(-1).copy()
@@ -66,7 +67,7 @@ def oracle_job(job):
 def single_lattice_job(job):
     """The same job on the single-lattice GPU path (lgca_create .. lgca_step)."""
     from lgca_amd import Lattice
-    lat = Lattice(int(job.model), dim_x=job.dim_x, dim_y=job.dim_y,
+    lat = Lattice(model_id(job.model), dim_x=job.dim_x, dim_y=job.dim_y,
                   coarse_graining_radius=job.coarse_radius, seed=job.seed)
     try:
         lat.set_cell_types(job_types(job))
@@ -278,32 +279,21 @@ def test_continuous_batching_mid_run_arrivals():
     check_job("long", got[lid], single_lattice_job(long_job))
 
 
-def test_c4_like_ensemble_matches_single_lattice_path():
-    """A slice of the 512-job ensemble: FHP-II jobs with long side 128..2048
-    (step 64), aspect 1..4, periodic / pipe / cylinder, fill 0.1-0.5, own seed;
-    steps cut to 60-200 so the comparison runs in seconds."""
-    from lgca_amd import Job
-    rng = np.random.default_rng(4)
-    jobs = []
-    for i in range(40):
-        L = int(rng.integers(2, 33)) * 64
-        a = float(rng.uniform(1.0, 4.0))
-        S = max(64, int(round(L / a / 64.0)) * 64)
-        geom = ["periodic", "pipe", "cylinder"][i % 3]
-        dx, dy = (L, S) if i % 2 == 0 else (S, L)
-        rad = int(rng.integers(8, max(9, min(dx, dy) // 8 + 1)))
-        jobs.append(Job(2, dx, dy, int(rng.integers(60, 201)), seed=1000 + i,
-                        fill=float(rng.uniform(0.1, 0.5)), geometry=geom, cyl_x=dx // 4,
-                        cyl_y=dy // 2, cyl_diameter=2.0 * rad, coarse_radius=16))
-    say(f"[batch] c4-like slice, {len(jobs)} jobs")
-    got, stats = run_batch(jobs)
+def test_ensemble_matches_single_lattice_path():
+    """The whole 512-job ensemble workload (FHP-II, long side 128..2048 step
+    64, aspect 1..4, periodic / pipe / cylinder, fill 0.1-0.5, own seeds), with
+    steps cut to 40-120 so the comparison runs in seconds: every job's state
+    and coarse fields equal its run alone on the single-lattice path, and the
+    eight smallest the oracle's."""
+    from lgca_amd.workloads import ensemble_jobs
+    jobs = ensemble_jobs(512, steps_range=(40, 121))
+    say(f"[batch] ensemble workload, {len(jobs)} jobs")
+    got, stats = run_batch(jobs, budget=200 * 10 ** 9)
     say(f"[batch] stats {stats}")
     for i, j in enumerate(jobs):
-        check_job(f"c4 job {i} {j.dim_x}x{j.dim_y} {j.geometry}", got[i], single_lattice_job(j))
-    # and a few small ones against the oracle directly
-    for i in (0, 1, 2):
-        if jobs[i].num_cells * jobs[i].steps <= 3e8:
-            check_job(f"c4 job {i} vs oracle", got[i], oracle_job(jobs[i]))
+        check_job(f"ensemble job {i} {j.dim_x}x{j.dim_y} {j.geometry}", got[i], single_lattice_job(j))
+    for i in sorted(range(len(jobs)), key=lambda i: jobs[i].dim_x * jobs[i].dim_y)[:8]:
+        check_job(f"ensemble job {i} vs oracle", got[i], oracle_job(jobs[i]))
 
 
 def test_invalid_job_descriptions():
@@ -366,7 +356,7 @@ def test_serving_stream_matches_single_lattice_path():
     say(f"[serving] stats {stats}")
     assert stats["jobs_done"] == len(jobs)
     for i, job in enumerate(jobs):
-        name = f"job {i} m{int(job.model)} {job.dim_x}x{job.dim_y} {job.geometry} {job.steps} steps"
+        name = f"job {i} m{model_id(job.model)} {job.dim_x}x{job.dim_y} {job.geometry} {job.steps} steps"
         check_job(name, got[i], single_lattice_job(job))
     for i in sorted(range(len(jobs)), key=lambda i: jobs[i].dim_x * jobs[i].dim_y)[:4]:
         check_job(f"job {i} vs oracle", got[i], oracle_job(jobs[i]))
@@ -393,5 +383,5 @@ def test_new_job_in_a_freed_slot_gets_its_own_work_items():
             b.wait(ids[-1], timeout=60)  # each runs alone, in the slot the last one freed
         got = [b.result(i) for i in ids]
     for i, job in enumerate([first] + later):
-        say(f"[slot reuse] job {i} m{int(job.model)} {job.dim_x}x{job.dim_y}")
+        say(f"[slot reuse] job {i} m{model_id(job.model)} {job.dim_x}x{job.dim_y}")
         check_job(f"job {i}", got[i], oracle_job(job))
