@@ -24,6 +24,44 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ constexpr int num_dir_of_kind(int kind) { return kind == kHPP ? 4 : (kind == kFHP1 ? 6 : 7); }
 
+// ------------------------------------------------------- three-input logic
+// Any bitwise function of three words is one v_bitop3_b32 on gfx950.  Its
+// truth table LUT holds f(x, y, z) at bit 4x + 2y + z, so writing f on the
+// operand masks kX, kY, kZ gives the table: bit3<kX & ~kY & kZ>(x, y, z) is
+// x & ~y & z.  Left to itself the compiler splits many such expressions into
+// two-input ops; the rules below are written directly as three-input steps.
+constexpr uint32_t kX = 0xF0u, kY = 0xCCu, kZ = 0xAAu;
+
+template <uint32_t LUT>
+__host__ __device__ __forceinline__ uint32_t bit3(uint32_t x, uint32_t y, uint32_t z)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(x, y, z, (unsigned char)(LUT & 0xFFu));
+#else
+    uint32_t r = 0;
+    for (int i = 0; i < 8; ++i)
+        if ((LUT >> i) & 1u) r |= ((i & 4) ? x : ~x) & ((i & 2) ? y : ~y) & ((i & 1) ? z : ~z);
+    return r;
+#endif
+}
+
+template <uint32_t LUT>
+__host__ __device__ __forceinline__ u32x4 bit3(u32x4 x, u32x4 y, u32x4 z)
+{
+    u32x4 r;
+    r.x = bit3<LUT>(x.x, y.x, z.x);
+    r.y = bit3<LUT>(x.y, y.y, z.y);
+    r.z = bit3<LUT>(x.z, y.z, z.z);
+    r.w = bit3<LUT>(x.w, y.w, z.w);
+    return r;
+}
+
+constexpr uint32_t kXor3 = kX ^ kY ^ kZ;
+constexpr uint32_t kMaj = (kX & kY) | (kZ & (kX | kY));
+constexpr uint32_t kAnd3 = kX & kY & kZ;
+constexpr uint32_t kOr3 = kX | kY | kZ;
+constexpr uint32_t kMux = (kX & kY) | (~kX & kZ);  // x ? y : z
+
 // --------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, SC'11.  The key schedule is wave-uniform, so the
 // per-round Weyl increments stay in scalar registers.
@@ -37,8 +75,11 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, u
         if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        // the first two rounds often work on wave-uniform counter words,
+        // which the compiler keeps in scalar registers; later rounds use
+        // one three-input xor each
+        const uint32_t n0 = r < 2 ? ((uint32_t)(p1 >> 32) ^ c1 ^ k0) : bit3<kXor3>((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = r < 2 ? ((uint32_t)(p0 >> 32) ^ c3 ^ k1) : bit3<kXor3>((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -49,11 +90,13 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, u
 
 // ------------------------------------------------------------------- collision
 // HPP (src/lgca_models.h:134-152): an isolated head-on pair along one axis
-// turns into a pair along the other.
+// turns into a pair along the other, which happens exactly when s0 == s2,
+// s1 == s3 and s0 != s1.
 template <typename T>
 __host__ __device__ __forceinline__ void collide_hpp(T s[4])
 {
-    const T h = (s[0] & s[2] & ~(s[1] | s[3])) | (s[1] & s[3] & ~(s[0] | s[2]));
+    const T k = bit3<~(kX ^ kY) & (kX ^ kZ)>(s[0], s[2], s[1]);  // s0 == s2 != s1
+    const T h = bit3<kX & ~(kY ^ kZ)>(k, s[1], s[3]);             // and s1 == s3
     s[0] ^= h; s[1] ^= h; s[2] ^= h; s[3] ^= h;
 }
 
@@ -64,46 +107,48 @@ __host__ __device__ __forceinline__ void collide_hpp(T s[4])
 //   * exactly three, {0,2,4} or {1,3,5}: swap, rest untouched;
 //   * FHP-II: rest + exactly one moving particle i -> particles i-1 and i+1;
 //     exactly two moving at i-1 and i+1 without rest -> rest + particle i.
-// The occupancy count comes from a bit-sliced adder tree.
+// Every step is one three-input operation (43 for FHP-II, 25 for FHP-I).
 template <bool WITH_REST, typename T>
 __host__ __device__ __forceinline__ void collide_fhp(T s[7], T p)
 {
     const T a0 = s[0], a1 = s[1], a2 = s[2], a3 = s[3], a4 = s[4], a5 = s[5];
-    // 6-input population count: count = ones + 2 * twos + 4 * fours
-    const T x1 = a0 ^ a1 ^ a2, c1 = (a0 & a1) | (a2 & (a0 ^ a1));
-    const T x2 = a3 ^ a4 ^ a5, c2 = (a3 & a4) | (a5 & (a3 ^ a4));
-    const T ones = x1 ^ x2, cx = x1 & x2;
-    const T twos = c1 ^ c2 ^ cx, fours = (c1 & c2) | (cx & (c1 ^ c2));
-    const T o2 = ~ones & twos & ~fours;  // exactly two moving
-    const T o3 = ones & twos & ~fours;   // exactly three moving
-    // head-on pairs and symmetric triples
-    const T h0 = o2 & a0 & a3, h1 = o2 & a1 & a4, h2 = o2 & a2 & a5;
-    const T t = o3 & ((a0 & a2 & a4) | (a1 & a3 & a5));
+    // occupancy n = x1 + x2 + 2 (c1 + c2), from the sums of the two triples
+    const T x1 = bit3<kXor3>(a0, a1, a2), c1 = bit3<kMaj>(a0, a1, a2);
+    const T x2 = bit3<kXor3>(a3, a4, a5), c2 = bit3<kMaj>(a3, a4, a5);
+    const T u = x1 ^ x2;  // n odd
+    const T v = c1 ^ c2;  // exactly one triple holds two or more
+    // n == 2: x1 == x2 and either both are set with no carries, or neither
+    // is and exactly one carry is
+    const T w = bit3<(kX & ~(kY | kZ)) | (~kX & (kY ^ kZ))>(x1, c1, c2);
+    const T o2 = bit3<~kX & kY>(u, w, w);
+    // n == 3 with a0 == a2 == a4: the symmetric triples
+    const T eq = bit3<~(kX ^ kY) & ~(kY ^ kZ)>(a0, a2, a4);
+    const T t = bit3<kAnd3>(u, v, eq);
+    // head-on pairs with nothing else moving
+    const T h0 = bit3<kAnd3>(o2, a0, a3), h1 = bit3<kAnd3>(o2, a1, a4), h2 = bit3<kAnd3>(o2, a2, a5);
     // class m = d mod 3 flips when its own pair leaves or a pair rotates in:
     // p = 0 moves pair k to class k+1, p = 1 to class k+2.
-    const T A0 = t | h0 | (p & h1) | (~p & h2);
-    const T A1 = t | h1 | (p & h2) | (~p & h0);
-    const T A2 = t | h2 | (p & h0) | (~p & h1);
+    const T A0 = bit3<kOr3>(t, h0, bit3<kMux>(p, h1, h2));
+    const T A1 = bit3<kOr3>(t, h1, bit3<kMux>(p, h2, h0));
+    const T A2 = bit3<kOr3>(t, h2, bit3<kMux>(p, h0, h1));
     if (WITH_REST) {
         const T r = s[6];
-        const T o1 = ones & ~twos & ~fours;  // exactly one moving
-        const T e = o1 & r;                  // rest + lone particle
-        const T g = o2 & ~r;                 // two moving, no rest
-        // u_i: a rest-rule event centred on direction i (lone particle at i, or
-        // the pair at i-1, i+1); it flips directions i-1, i, i+1 and the rest bit.
-        const T u0 = (e & a0) | (g & a5 & a1);
-        const T u1 = (e & a1) | (g & a0 & a2);
-        const T u2 = (e & a2) | (g & a1 & a3);
-        const T u3 = (e & a3) | (g & a2 & a4);
-        const T u4 = (e & a4) | (g & a3 & a5);
-        const T u5 = (e & a5) | (g & a4 & a0);
-        s[0] = a0 ^ (A0 | u5 | u0 | u1);
-        s[1] = a1 ^ (A1 | u0 | u1 | u2);
-        s[2] = a2 ^ (A2 | u1 | u2 | u3);
-        s[3] = a3 ^ (A0 | u2 | u3 | u4);
-        s[4] = a4 ^ (A1 | u3 | u4 | u5);
-        s[5] = a5 ^ (A2 | u4 | u5 | u0);
-        s[6] = r ^ (u0 | u1 | u2 | u3 | u4 | u5);
+        const T o1 = bit3<kX & ~(kY | kZ)>(u, c1, c2);  // n == 1
+        // n == 2 with both particles in one of the triples {0,2,4}, {1,3,5}:
+        // the pair i-1, i+1 of the rest rule
+        const T y = bit3<kX & (kY | kZ)>(o2, bit3<kMaj>(a0, a2, a4), bit3<kMaj>(a1, a3, a5));
+        // a rest-rule event: with rest, a lone particle i moves to i-1 and
+        // i+1, so direction d takes a[d-1] | a[d+1]; without, the pair i-1,
+        // i+1 merges into i, so d takes a[d-1] & a[d+1].  Both are
+        // maj(r, a[d-1], a[d+1]), and the rest bit flips.
+        const T Z = bit3<kMux>(r, o1, y);
+        s[0] = bit3<kMux>(Z, bit3<kMaj>(r, a5, a1), a0) ^ A0;
+        s[1] = bit3<kMux>(Z, bit3<kMaj>(r, a0, a2), a1) ^ A1;
+        s[2] = bit3<kMux>(Z, bit3<kMaj>(r, a1, a3), a2) ^ A2;
+        s[3] = bit3<kMux>(Z, bit3<kMaj>(r, a2, a4), a3) ^ A0;
+        s[4] = bit3<kMux>(Z, bit3<kMaj>(r, a3, a5), a4) ^ A1;
+        s[5] = bit3<kMux>(Z, bit3<kMaj>(r, a4, a0), a5) ^ A2;
+        s[6] = r ^ Z;
     } else {
         s[0] = a0 ^ A0; s[1] = a1 ^ A1; s[2] = a2 ^ A2;
         s[3] = a3 ^ A0; s[4] = a4 ^ A1; s[5] = a5 ^ A2;
