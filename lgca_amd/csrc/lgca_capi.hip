@@ -37,6 +37,15 @@ struct LaunchShape {
     int k;
     uint32_t th;
 };
+// A strip-mode dispatch order in device memory (tile_wave_order), for passes
+// of k steps, th-row tiles and halo `halo` under the cell types and forcing
+// that `key` names.
+struct WaveOrder {
+    int k;
+    uint32_t th, halo;
+    uint64_t key;
+    uint32_t* dev;
+};
 // steps_per_pass value that lets the trial choose it
 constexpr int kAutoSteps = -1;
 }  // namespace lgca
@@ -54,6 +63,9 @@ struct lgca_lattice {
     uint32_t* snap = nullptr;
     uint32_t* mask = nullptr;       // allocated once any non-fluid cell is set
     int32_t* row_range = nullptr;
+    std::vector<int32_t> row_lo, row_hi;  // host copy of row_range (empty without solids)
+    uint64_t types_version = 0;     // counts cell-type uploads
+    std::vector<WaveOrder> orders;  // cached dispatch orders
     uint32_t* dec = nullptr;        // force decisions, [k][rows][fwn]
     size_t dec_cap = 0;             // words
     uint64_t dec_step0 = 0;
@@ -389,6 +401,7 @@ extern "C" lgca_status lgca_destroy(lgca_lattice* h)
                         h->velocity, h->bytes, h->visited, h->counters, h->ctrl};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
+        for (const WaveOrder& o : h->orders) (void)hipFree(o.dev);
         for (hipEvent_t ev : h->trial_ev) (void)hipEventDestroy(ev);
         if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     }
@@ -497,6 +510,15 @@ static lgca_status upload_types(lgca_lattice* h, const uint8_t* types, hipMemcpy
     unsigned long long flag = 0;
     if ((st = read_counters(h, &flag, 1))) return st;
     if (flag & 1ull) return fail(h, LGCA_ERR_INVALID_STATE, "cell types must be 0, 1 or 2");
+    // the rows' solid word ranges on the host, for the dispatch order; the
+    // stream is idle here, so the cached orders can go
+    ++h->types_version;
+    for (const WaveOrder& o : h->orders) (void)hipFree(o.dev);
+    h->orders.clear();
+    h->row_lo.resize(h->dim_y);
+    h->row_hi.resize(h->dim_y);
+    HIPCHK(h, hipMemcpy(h->row_lo.data(), h->row_range, (size_t)h->dim_y * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(h->row_hi.data(), h->row_range + h->dim_y, (size_t)h->dim_y * 4, hipMemcpyDeviceToHost));
     return LGCA_OK;
 }
 
@@ -704,10 +726,10 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
     const uint64_t wave_rows = tile_segments(h->g, k) ? (uint64_t)h->dim_y / (64 / h->g.W)
                                                       : (uint64_t)h->dim_y * tile_strips(h->g, k);
     // about 2.1 rounds of resident waves (four per SIMD on 1024 SIMDs); on the
-    // 32768 x 8192 lattice this gives 16-row tiles, the fastest measured with
-    // the XCD-contiguous block order.  Small lattices get short tiles: a
-    // wave's walk of th + 2K rows is then their critical path (2-row tiles run
-    // the 2048 x 512 pipe 1.6x faster than 8)
+    // 32768 x 8192 lattice this gives 16-row tiles, the fastest 4-step shape
+    // there.  Small lattices get short tiles: a wave's walk of th + 2K rows is
+    // then their critical path (2-row tiles run the 2048 x 512 pipe 1.6x
+    // faster than 8)
     uint64_t th = wave_rows / 8704;
     if (th < 2) th = 2;
     if (th > 256) th = 256;
@@ -732,17 +754,18 @@ static int steps_for_launch(int want, uint64_t left)
 // (-1: none).  With automatic rows, the first launches run candidate shapes,
 // each twice in a group of consecutive launches between two events on the
 // lattice's stream: at the configured steps per pass (4 when automatic) the
-// rule's height, then 4, 8, 12, 16, 20, 24, 32 rows; and, with automatic
-// steps on a lattice the rule gives 2-row tiles, 2-step passes of 2 and 4
-// rows, whose shorter walks suit lattices too small to fill the GPU (the
-// 2048 x 512 pipe runs 10 % faster so).  Groups are single launches, or four
+// rule's height, then 4, 8, 12, 16, 20, 24, 32 rows; with automatic steps on
+// a lattice that fills the GPU (the rule's height is 4 rows or more), 8-step
+// passes of 16 to 64 rows, which move the state through HBM half as often
+// (the 32768 x 8192 Karman lattice runs 8-step passes of 40 rows 1.4x faster
+// than 4-step passes of 16), and on a lattice the rule gives 2-row tiles,
+// 2-step passes of 2 and 4 rows, whose shorter walks suit lattices too
+// small to fill the GPU (the 2048 x 512 pipe runs 10 % faster so).  Groups are single launches, or four
 // when 2-step passes compete: then the gaps between launches must count as
 // they do in a long run, and timing each launch alone penalised the shorter
 // 2-step launches.  Once the last group has finished (queried, never waited
 // for) the fastest per step is kept, the rule's unless another beats it by
-// more than 2 %.  The rule is fixed on the 32768 x 8192 lattice, where its 16
-// rows measured best, but other shapes gain up to a fifth from taller or
-// shorter tiles.  Results do not depend on the shape.
+// more than 2 %.  Results do not depend on the shape.
 constexpr uint32_t kTrialReps = 2;
 
 // The trial's candidate shapes, the rule's own first.
@@ -761,6 +784,8 @@ static std::vector<LaunchShape> trial_candidates(const lgca_lattice* h)
         add(2, 2);
         add(2, 4);
     }
+    if (h->tile_k == kAutoSteps && c[0].th >= 4)
+        for (uint32_t r : {16u, 24u, 32u, 40u, 48u, 64u}) add(8, r);
     return c;
 }
 
@@ -776,7 +801,9 @@ static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t*
     }
     if (!h->tuned.th && h->trial.empty()) {
         h->trial = trial_candidates(h);
-        h->trial_group = h->trial.back().k == 2 && base_k != 2 ? 4 : 1;
+        h->trial_group = 1;
+        for (const LaunchShape& s : h->trial)
+            if (s.k == 2 && base_k != 2) h->trial_group = 4;
         h->trial_issued = 0;
         h->trial_open = -1;
         const size_t groups = kTrialReps * h->trial.size();
@@ -881,6 +908,40 @@ static TileArgs with_k(TileArgs a, int k)
     return a;
 }
 
+// The dispatch order of a strip-mode pass of k steps with arguments a, built
+// and cached on first use; null in segment mode, when it cannot be built, or
+// when it is not cached and may_alloc is false (the kernel then uses its own
+// fixed order; results never depend on the order).
+static const uint32_t* wave_order(lgca_lattice* h, int k, const TileArgs& a, bool may_alloc)
+{
+    if (tile_segments(h->g, k)) return nullptr;
+    uint64_t key = h->types_version;
+    for (uint64_t v : {(uint64_t)(a.fdec != nullptr), (uint64_t)a.fw0, (uint64_t)a.fwn, (uint64_t)a.fy0,
+                       (uint64_t)a.fy1})
+        key = key * 0x100000001B3ull ^ v;
+    for (const WaveOrder& o : h->orders)
+        if (o.k == k && o.th == a.th && o.halo == a.halo && o.key == key) return o.dev;
+    if (!may_alloc) return nullptr;
+    std::vector<uint32_t> v;
+    const bool solids = h->mask && h->row_lo.size() == h->dim_y;
+    if (!tile_wave_order(a, k, solids ? h->row_lo.data() : nullptr, solids ? h->row_hi.data() : nullptr, v))
+        return nullptr;
+    if (h->orders.size() >= 16) {
+        // the oldest may still be read by queued passes
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+        (void)hipFree(h->orders.front().dev);
+        h->orders.erase(h->orders.begin());
+    }
+    uint32_t* dev = nullptr;
+    if (dalloc(&dev, v.size() * 4) != hipSuccess) return nullptr;
+    if (hipMemcpy(dev, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dev);
+        return nullptr;
+    }
+    h->orders.push_back(WaveOrder{k, a.th, a.halo, key, dev});
+    return dev;
+}
+
 static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_dec = nullptr,
                                size_t ext_words = 0);
 
@@ -928,10 +989,11 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
                 const TileArgs a = with_k(tile_args(h, cand[i].th, fdec), cand[i].k);
                 const int n = 8 / cand[i].k;
                 // the first pass of a shape loads its code: untimed
-                if (rep == 0 && fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream), "launch_tile")) break;
+                const uint32_t* ord = wave_order(h, cand[i].k, a, true);
+                if (rep == 0 && fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream, ord), "launch_tile")) break;
                 if (fail_hip(hipEventRecord(ev[0], h->stream), "hipEventRecord")) break;
                 for (int l = 0; l < n && st == LGCA_OK; ++l)
-                    fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream), "launch_tile");
+                    fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream, ord), "launch_tile");
                 if (st || fail_hip(hipEventRecord(ev[1], h->stream), "hipEventRecord") ||
                     fail_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize"))
                     break;
@@ -955,7 +1017,7 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
             for (uint32_t o = 1; o < 4 && k > 1 && st == LGCA_OK; ++o) {
                 TileArgs a = with_k(tile_args(h, h->tuned.th, fdec ? fdec + o * per_step : nullptr), k);
                 a.t0 += o;
-                fail_hip(launch_tile(h->kind, k, a, h->stream), "launch_tile");
+                fail_hip(launch_tile(h->kind, k, a, h->stream, wave_order(h, k, a, true)), "launch_tile");
             }
             if (st == LGCA_OK) fail_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         }
@@ -1101,7 +1163,8 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_d
                 h->trial_open = ev_start;
                 h->trial_steps[ev_start] = 0;
             }
-            HIPCHK(h, launch_tile(h->kind, k, with_k(tile_args(h, th, fdec), k), h->stream));
+            const TileArgs a = with_k(tile_args(h, th, fdec), k);
+            HIPCHK(h, launch_tile(h->kind, k, a, h->stream, wave_order(h, k, a, ext_dec == nullptr)));
             if (h->trial_open >= 0) h->trial_steps[h->trial_open] += (uint32_t)k;
             if (ev_end >= 0) {
                 HIPCHK(h, hipEventRecord(h->trial_ev[2 * ev_end + 1], h->stream));

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace lgca {
 
 struct Geometry {
@@ -62,17 +64,27 @@ uint32_t tile_strips(const Geometry& g, int k);  // x-strips per row tile in str
 // Segment mode: rows of W <= 64 words, S = 64 / W whole rows per wave (one per
 // row tile), used when it beats strips (S >= 2, or a row needing two strips).
 bool tile_segments(const Geometry& g, int k);
-hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s);
+// A pass of k steps; `order` is a dispatch order from tile_wave_order for the
+// same arguments and k, in device memory, or null.
+hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order = nullptr);
+// Strip-mode dispatch order of a pass (one entry per wave, tile * strips +
+// strip), from the rows' non-fluid word ranges row_lo / row_hi on the host
+// (null when the lattice has no solids).  False in segment mode, where the
+// order is fixed.
+bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int32_t* row_hi,
+                     std::vector<uint32_t>& order);
 
-// Strip-mode block order of tile_kernel: the heavy block columns (those with
-// a forced strip, at most kMaxHeavy, ascending) of every row tile are
-// dispatched before the others (nheavy 0: plain order).
-constexpr uint32_t kMaxHeavy = 8;
+// Strip-mode wave order of tile_kernel (one wave per workgroup, one per row
+// tile and strip): entry blockIdx.x of `order` when there is one, otherwise
+// the heavy strips (those over forced words, at most kMaxHeavy, ascending) of
+// every row tile before the others (nheavy 0: plain order).
+constexpr uint32_t kMaxHeavy = 32;
 struct TileOrder {
-    uint32_t nbx;               // block columns (four strips each)
+    uint32_t nstrips;           // strips per row tile
     uint32_t ntiles;            // row tiles
-    uint32_t nheavy;            // heavy block columns
+    uint32_t nheavy;            // heavy strips
     uint32_t heavy[kMaxHeavy];  // their indices, ascending
+    const uint32_t* order;      // device order (tile_wave_order), or null
 };
 
 // Batched ensembles.  BatchJob is a job's static descriptor (its TileArgs
@@ -109,12 +121,11 @@ hipError_t launch_tile_batch(int k, bool ragged, const BatchJob* jobs, BatchStat
 // by lgca_tile_inst.hip; grid and shapes come checked from launch_tile /
 // launch_tile_batch.
 template <int KIND, int K>
-hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblocks, bool ragged,
+hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves, bool ragged,
                           bool seg, hipStream_t s);
 template <int K>
 hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
-                                uint32_t n_items, uint32_t nblocks, uint32_t heavy_blocks, bool ragged,
-                                hipStream_t s);
+                                uint32_t n_items, uint32_t n_heavy, bool ragged, hipStream_t s);
 
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]

@@ -1,10 +1,12 @@
 // lgca_tile.hip -- host side of the temporally blocked step: launch shapes,
-// grids and the strip-mode block order, dispatched to the per-(model, K) and
+// grids and the strip-mode wave order, dispatched to the per-(model, K) and
 // per-K batch launchers that lgca_tile_inst.hip instantiates from lgca_tile.hpp.
 #include "lgca_dev.hpp"
 #include "lgca_kernels.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 
 namespace lgca {
 
@@ -12,12 +14,12 @@ hipError_t launch_tile_batch(int k, bool ragged, const BatchJob* jobs, BatchStat
                              const WorkItem* items, uint32_t n_items, uint32_t n_heavy, hipStream_t s)
 {
     if (!n_items) return hipSuccess;
-    const uint32_t nblk = (n_items + 3) / 4, hblk = std::min<uint32_t>((n_heavy + 3) / 4, nblk);
+    n_heavy = std::min(n_heavy, n_items);
     switch (k) {
-    case 1: return launch_tile_batch_kk<1>(jobs, state, items, n_items, nblk, hblk, ragged, s);
-    case 2: return launch_tile_batch_kk<2>(jobs, state, items, n_items, nblk, hblk, ragged, s);
-    case 4: return launch_tile_batch_kk<4>(jobs, state, items, n_items, nblk, hblk, ragged, s);
-    case 8: return launch_tile_batch_kk<8>(jobs, state, items, n_items, nblk, hblk, ragged, s);
+    case 1: return launch_tile_batch_kk<1>(jobs, state, items, n_items, n_heavy, ragged, s);
+    case 2: return launch_tile_batch_kk<2>(jobs, state, items, n_items, n_heavy, ragged, s);
+    case 4: return launch_tile_batch_kk<4>(jobs, state, items, n_items, n_heavy, ragged, s);
+    case 8: return launch_tile_batch_kk<8>(jobs, state, items, n_items, n_heavy, ragged, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -39,51 +41,114 @@ bool tile_segments(const Geometry& g, int k)
     return g.W <= 64 && (64 / g.W >= 2 || tile_strips(g, k) >= 2);
 }
 
-hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s)
+namespace {
+
+// Does strip st (words [st * stride - halo, + 64) modulo W) touch words
+// [lo, hi) of its rows?
+bool strip_hits(const Geometry& g, uint32_t stride, uint32_t halo, uint32_t st, int64_t lo, int64_t hi)
+{
+    if (lo >= hi) return false;
+    const int64_t W = g.W;
+    if (W <= 64) return true;  // the strip wraps the whole row
+    const int64_t a0 = (((int64_t)st * stride - halo) % W + W) % W;
+    for (int64_t k = -1; k <= 1; ++k)
+        if (a0 < hi + k * W && a0 + 64 > lo + k * W) return true;
+    return false;
+}
+
+// Appends v in the order that hands each XCD a contiguous run of it:
+// workgroups go to the 8 XCDs round-robin, so entry b of the appended range
+// runs on XCD (start + b) mod 8.
+void append_xcd_runs(std::vector<uint32_t>& order, const std::vector<uint32_t>& v)
+{
+    const uint32_t n = (uint32_t)v.size(), q = n >> 3, r = n & 7u;
+    for (uint32_t b = 0; b < n; ++b) {
+        const uint32_t xcd = b & 7u;
+        order.push_back(v[xcd * q + std::min(xcd, r) + (b >> 3)]);
+    }
+}
+
+}  // namespace
+
+bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int32_t* row_hi,
+                     std::vector<uint32_t>& order)
+{
+    order.clear();
+    if (tile_segments(a.g, k) || a.th < 2) return false;
+    const uint32_t stride = 64u - 2u * a.halo;
+    const uint32_t nstrips = (a.g.W + stride - 1) / stride;
+    const uint32_t ntiles = (a.g.dim_y + a.th - 1) / a.th;
+    if ((uint64_t)nstrips * ntiles > ((uint64_t)1 << 24)) return false;
+    const bool forced = a.fdec && a.fwn;
+    std::vector<uint8_t> fstrip(nstrips, 0);
+    if (forced)
+        for (uint32_t st = 0; st < nstrips; ++st)
+            fstrip[st] = strip_hits(a.g, stride, a.halo, st, a.fw0, (int64_t)a.fw0 + a.fwn);
+    // the cost classes: walks over solids (about twice a plain walk), over
+    // forced words (about a sixth more), plain
+    std::vector<uint32_t> solid, force, plain;
+    const int64_t dy = a.g.dim_y;
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        const int64_t y0 = (int64_t)t * a.th, rows = std::min<int64_t>(a.th, dy - y0);
+        int64_t lo = INT64_MAX, hi = INT64_MIN;  // hull of the walk rows' solid words
+        bool frow = false;
+        for (int64_t y = y0 - k; y < y0 + rows + k; ++y) {
+            const int64_t r = ((y % dy) + dy) % dy;
+            if (row_lo && row_lo[r] < row_hi[r]) {
+                lo = std::min<int64_t>(lo, row_lo[r]);
+                hi = std::max<int64_t>(hi, row_hi[r]);
+            }
+            frow |= r >= (int64_t)a.fy0 && r < (int64_t)a.fy1;
+        }
+        for (uint32_t st = 0; st < nstrips; ++st) {
+            const uint32_t it = t * nstrips + st;
+            if (lo < hi && strip_hits(a.g, stride, a.halo, st, lo, hi)) solid.push_back(it);
+            else if (forced && frow && fstrip[st]) force.push_back(it);
+            else plain.push_back(it);
+        }
+    }
+    // The few long solid walks go first, spread over the XCDs; then the
+    // forced and the plain ones, each in contiguous runs per XCD, so that
+    // the tiles an XCD holds are vertical neighbours whose halo rows meet in
+    // its L2.
+    order.reserve((size_t)nstrips * ntiles);
+    order.insert(order.end(), solid.begin(), solid.end());
+    append_xcd_runs(order, force);
+    append_xcd_runs(order, plain);
+    return true;
+}
+
+hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order)
 {
     const bool seg = tile_segments(a.g, k);
     if (!seg && (a.halo < tile_halo(a.g, k) || a.halo > 16)) return hipErrorInvalidValue;
     if (a.th < 2 || (a.th & 1u) || a.th > 65535) return hipErrorInvalidValue;
     const uint32_t ntiles = (a.g.dim_y + a.th - 1) / a.th;
-    dim3 grid;
+    uint32_t nwaves = 0;  // one wave per workgroup
     TileOrder o{};
     if (seg) {
-        const uint32_t groups = (ntiles + 64 / a.g.W - 1) / (64 / a.g.W);
-        grid = dim3((groups + 3) / 4, 1);
+        nwaves = (ntiles + 64 / a.g.W - 1) / (64 / a.g.W);
     } else {
         const uint32_t stride = 64u - 2u * a.halo;
         const uint32_t nstrips = (a.g.W + stride - 1) / stride;
-        o.nbx = (nstrips + 3) / 4;
+        o.nstrips = nstrips;
         o.ntiles = ntiles;
-        if ((uint64_t)o.nbx * ntiles >= ((uint64_t)1 << 31)) return hipErrorInvalidValue;
-        grid = dim3(o.nbx * ntiles);
-        // heavy block columns: those with a strip over a forced word; strip
-        // st spans words [st * stride - halo, + 64) modulo W
-        if (a.fdec && a.fwn && nstrips <= 4096) {
-            const int64_t W = a.g.W;
-            auto hits_force = [&](uint32_t st) {
-                if (W <= 64) return true;  // the strip wraps the whole row
-                const int64_t a0 = (((int64_t)st * stride - a.halo) % W + W) % W;
-                for (int64_t k = -1; k <= 1; ++k) {
-                    const int64_t b0 = (int64_t)a.fw0 + k * W, b1 = b0 + a.fwn;
-                    if (a0 < b1 && a0 + 64 > b0) return true;
-                }
-                return false;
-            };
-            for (uint32_t bx = 0; bx < o.nbx; ++bx) {
-                bool heavy = false;
-                for (uint32_t st = 4 * bx; st < 4 * bx + 4 && st < nstrips; ++st) heavy |= hits_force(st);
-                if (!heavy) continue;
+        if ((uint64_t)nstrips * ntiles >= ((uint64_t)1 << 31)) return hipErrorInvalidValue;
+        nwaves = nstrips * ntiles;
+        o.order = order;
+        // without an order: the heavy strips are those over a forced word
+        if (!order && a.fdec && a.fwn && nstrips <= 16384) {
+            for (uint32_t st = 0; st < nstrips; ++st) {
+                if (!strip_hits(a.g, stride, a.halo, st, a.fw0, (int64_t)a.fw0 + a.fwn)) continue;
                 if (o.nheavy == kMaxHeavy) { o.nheavy = 0; break; }  // many: plain order
-                o.heavy[o.nheavy++] = bx;
+                o.heavy[o.nheavy++] = st;
             }
-            if (o.nheavy == o.nbx) o.nheavy = 0;
+            if (o.nheavy == nstrips) o.nheavy = 0;
         }
     }
     const bool ragged = a.g.nb != 32;
-    const uint32_t nblocks = grid.x;
 #define LGCA_TILE_CASE(KD, KK) \
-    if (kind == KD && k == KK) return launch_tile_kk<KD, KK>(a, o, nblocks, ragged, seg, s);
+    if (kind == KD && k == KK) return launch_tile_kk<KD, KK>(a, o, nwaves, ragged, seg, s);
     LGCA_TILE_CASE(kHPP, 1) LGCA_TILE_CASE(kHPP, 2) LGCA_TILE_CASE(kHPP, 4) LGCA_TILE_CASE(kHPP, 8)
     LGCA_TILE_CASE(kFHP1, 1) LGCA_TILE_CASE(kFHP1, 2) LGCA_TILE_CASE(kFHP1, 4) LGCA_TILE_CASE(kFHP1, 8)
     LGCA_TILE_CASE(kFHP2, 1) LGCA_TILE_CASE(kFHP2, 2) LGCA_TILE_CASE(kFHP2, 4) LGCA_TILE_CASE(kFHP2, 8)

@@ -583,45 +583,55 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     else walk(std::false_type{}, std::false_type{});
 }
 
-// Strip-mode block order (TileOrder): a block holds four consecutive strips
-// of one row tile; block columns are those four-strip groups.  Waves whose
-// strips carry forcing run longer, and dispatched in plain order the last of
-// them trail the rest of the grid; so the heavy block columns (of every row
-// tile) are dispatched first and the light ones fill in behind them.
+// One wave per workgroup.  A CU takes a new workgroup only when every SIMD the
+// workgroup would use has room, so with four waves per workgroup the waves
+// that end early (strips past the row end, or the light strips beside a
+// forced one) kept slots of a CU idle until the longest wave of their
+// workgroup ended: on the 32768 x 8192 Karman lattice half of the CUs ran
+// with 4 of their 16 wave slots busy for the first quarter of each launch.
+//
+// Strip-mode wave order (TileOrder): waves over solids or forced words run
+// longer, and dispatched in plain order the last of them trail the rest of
+// the grid; so they are dispatched first (tile_wave_order; without an order,
+// the forced strips of every row tile) and the light ones fill in behind
+// them.
 template <int KIND, int K, int O, bool RAGGED, bool SEG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_kernel(TileArgs a, TileOrder o)
 {
-    // segment mode: grid (tile groups / 4); strip mode: grid (nbx * ntiles)
+    // segment mode: grid (tile groups); strip mode: grid (nstrips * ntiles)
     if (SEG) {
-        const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-        tile_body<KIND, K, O, RAGGED, true>(a, 0, wave);
+        tile_body<KIND, K, O, RAGGED, true>(a, 0, blockIdx.x);
         return;
     }
     // Workgroups go to the 8 XCDs round-robin.  Each XCD takes a contiguous
     // run of the order instead, so the tiles it holds are vertical neighbours
-    // whose halo rows meet in its L2 (32768 x 8192 Karman: 6830 -> 6990
-    // GSUP/s at the best tile height of each mapping).  The heavy and the
-    // light blocks are mapped apart, so that every XCD gets its share of both
-    // (7086 -> 7452: mapped as one range, the forced blocks all landed on the
-    // first XCDs, which then finished last).
-    const uint32_t H = o.nheavy * o.ntiles;
-    const bool in_heavy = blockIdx.x < H;
-    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - H, n = in_heavy ? H : gridDim.x - H;
-    const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
-    const uint32_t L = (in_heavy ? 0u : H) + xcd * q + min(xcd, r) + (b >> 3);
-    uint32_t bx, tile;
-    if (L < H) {
-        tile = L / o.nheavy;
-        bx = o.heavy[L - tile * o.nheavy];
+    // whose halo rows meet in its L2.  The heavy and the light waves are
+    // mapped apart, so that every XCD gets its share of both (mapped as one
+    // range, the forced waves all land on the first XCDs, which then finish
+    // last).
+    uint32_t strip, tile;
+    if (o.order) {
+        const uint32_t it = o.order[blockIdx.x];
+        tile = it / o.nstrips;
+        strip = it - tile * o.nstrips;
     } else {
-        const uint32_t nl = o.nbx - o.nheavy, l = L - H;
-        tile = l / nl;
-        bx = l - tile * nl;  // the bx-th light column: step over the heavy ones
-        for (uint32_t h = 0; h < o.nheavy; ++h)
-            if (o.heavy[h] <= bx) ++bx;
+        const uint32_t H = o.nheavy * o.ntiles;
+        const bool in_heavy = blockIdx.x < H;
+        const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - H, n = in_heavy ? H : gridDim.x - H;
+        const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
+        const uint32_t L = (in_heavy ? 0u : H) + xcd * q + min(xcd, r) + (b >> 3);
+        if (L < H) {
+            tile = L / o.nheavy;
+            strip = o.heavy[L - tile * o.nheavy];
+        } else {
+            const uint32_t nl = o.nstrips - o.nheavy, l = L - H;
+            tile = l / nl;
+            strip = l - tile * nl;  // the strip-th light strip: step over the heavy ones
+            for (uint32_t h = 0; h < o.nheavy; ++h)
+                if (o.heavy[h] <= strip) ++strip;
+        }
     }
-    const uint32_t strip = __builtin_amdgcn_readfirstlane(bx * 4 + (threadIdx.x >> 6));
     tile_body<KIND, K, O, RAGGED, false, true>(a, strip, tile);
 }
 
@@ -635,20 +645,20 @@ void tile_kernel(TileArgs a, TileOrder o)
 // bit of an item's shape picks segment mode for its job, so strip and segment
 // jobs share the launch too.
 template <int K, bool RAGGED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
-                       const WorkItem* __restrict__ items, uint32_t n_items, uint32_t heavy_blocks)
+                       const WorkItem* __restrict__ items, uint32_t n_items, uint32_t n_heavy)
 {
-    // Blocks go to the 8 XCDs round-robin; the leading (heavy) blocks and the
-    // rest each map to contiguous runs per XCD, as in tile_kernel.
-    const bool in_heavy = blockIdx.x < heavy_blocks;
-    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - heavy_blocks;
-    const uint32_t n = in_heavy ? heavy_blocks : gridDim.x - heavy_blocks;
+    // One wave (workgroup) per item.  Workgroups go to the 8 XCDs
+    // round-robin; the leading (heavy) items and the rest each map to
+    // contiguous runs per XCD, as in tile_kernel.
+    const bool in_heavy = blockIdx.x < n_heavy;
+    const uint32_t b = in_heavy ? blockIdx.x : blockIdx.x - n_heavy;
+    const uint32_t n = in_heavy ? n_heavy : gridDim.x - n_heavy;
     const uint32_t xcd = b & 7u, q = n >> 3, r = n & 7u;
-    const uint32_t blk = (in_heavy ? 0u : heavy_blocks) + xcd * q + min(xcd, r) + (b >> 3);
     // wave-uniform by construction; readfirstlane lets the compiler keep the
     // job record in scalar registers
-    const uint32_t wi = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
+    const uint32_t wi = __builtin_amdgcn_readfirstlane((in_heavy ? 0u : n_heavy) + xcd * q + min(xcd, r) + (b >> 3));
     if (wi >= n_items) return;
     const WorkItem it = items[wi];
     const BatchState bs = state[it.job];
@@ -691,11 +701,11 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
 // Launchers of one (model, K) pair; lgca_tile.hip has validated the shape and
 // built the grid.  The pass offset O = t0 & 3 selects the instantiation.
 template <int KIND, int K>
-hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblocks, bool ragged,
+hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves, bool ragged,
                           bool seg, hipStream_t s)
 {
     const int off = K == 1 ? 0 : (int)(a.t0 & 3u);
-    const dim3 grid(nblocks), block(256);
+    const dim3 grid(nwaves), block(64);
 #define LGCA_TILE_LAUNCH(OO, RG, SG) \
     hipLaunchKernelGGL((tile_kernel<KIND, K, OO, RG, SG>), grid, block, 0, s, a, o)
 #define LGCA_TILE_OFF(OO)                                         \
@@ -722,15 +732,14 @@ hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nblock
 
 template <int K>
 hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
-                                uint32_t n_items, uint32_t nblocks, uint32_t heavy_blocks, bool ragged,
-                                hipStream_t s)
+                                uint32_t n_items, uint32_t n_heavy, bool ragged, hipStream_t s)
 {
     if (ragged)
-        hipLaunchKernelGGL((tile_batch_kernel<K, true>), dim3(nblocks), dim3(256), 0, s, jobs, state, items,
-                           n_items, heavy_blocks);
+        hipLaunchKernelGGL((tile_batch_kernel<K, true>), dim3(n_items), dim3(64), 0, s, jobs, state, items,
+                           n_items, n_heavy);
     else
-        hipLaunchKernelGGL((tile_batch_kernel<K, false>), dim3(nblocks), dim3(256), 0, s, jobs, state, items,
-                           n_items, heavy_blocks);
+        hipLaunchKernelGGL((tile_batch_kernel<K, false>), dim3(n_items), dim3(64), 0, s, jobs, state, items,
+                           n_items, n_heavy);
     return hipGetLastError();
 }
 

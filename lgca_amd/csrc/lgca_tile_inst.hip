@@ -16,7 +16,7 @@ template hipError_t launch_tile_kk<LGCA_INST_KIND, LGCA_INST_K>(const TileArgs&,
                                                                 bool, bool, hipStream_t);
 #else
 template hipError_t launch_tile_batch_kk<LGCA_INST_K>(const BatchJob*, BatchState*, const WorkItem*, uint32_t,
-                                                      uint32_t, uint32_t, bool, hipStream_t);
+                                                      uint32_t, bool, hipStream_t);
 #endif
 
 }  // namespace lgca

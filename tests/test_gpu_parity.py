@@ -350,11 +350,12 @@ def test_codec_round_trip_ragged(Lattice):
 
 def test_full_karman_size_through_the_default_path(Lattice):
     """FHP-III 32768x8192 with the cylinder of diameter 512 at (4096, 4099),
-    walls, fill 0.25, x force p = 2e-3 on x < 64, seed 11, r = 32: 96 steps
-    through the default launch path (the trial's 72 steps of candidate shapes,
-    then the kept shape) equal the oracle's bits, and so do the coarse fields
-    of the final state."""
-    dx, dy, steps = 32768, 8192, 96
+    walls, fill 0.25, x force p = 2e-3 on x < 64, seed 11, r = 32: 200 steps
+    through the default launch path (the trial's 152 steps of candidate
+    shapes, 4-step passes of 7 heights and 8-step passes of 6, then the kept
+    shape) equal the oracle's bits, and so do the coarse fields of the final
+    state."""
+    dx, dy, steps = 32768, 8192, 200
     say("[full] karman 32768x8192")
     lat = Lattice(3, dim_x=dx, dim_y=dy, coarse_graining_radius=32, seed=11)
     lat.apply_bc_cylinder(4096, 4099, 512.0)
@@ -366,16 +367,16 @@ def test_full_karman_size_through_the_default_path(Lattice):
     np.testing.assert_array_equal(state, want0)
     del want0
     n0 = lat.get_n_particles()
-    lat.step(76)
+    lat.step(160)
     lat.synchronize()
-    lat.step(20)  # decides on the trial, then runs the kept shape
+    lat.step(40)  # decides on the trial, then runs the kept shape
     info = lat.info()
     say(f"[full] kept steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
-    assert info.steps_per_pass == 4 and info.tile_rows > 0
+    assert info.steps_per_pass in (4, 8) and info.tile_rows > 0
     got = lat.get_state_u8()
     assert lat.get_n_particles() == n0
-    want = orc.run(3, dx, dy, types, state, 11, 0, steps, dict(dir="x", p=2e-3, x0=0, x1=64),
-                   nthreads=16)
+    want = orc.fast_run(3, dx, dy, types, state, 11, 0, steps, dict(dir="x", p=2e-3, x0=0, x1=64),
+                        nthreads=16)
     del state
     np.testing.assert_array_equal(got, want)
     del got
