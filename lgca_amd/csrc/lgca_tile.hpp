@@ -142,7 +142,7 @@ __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 }  // namespace
 
 // K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
-template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 3; };
+template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
 
 // One wave's work.  Strip mode (SEG false): x-strip `strip` of row tile
 // `tile`.  Segment mode (SEG true, rows of W <= 64 words): the wave's lanes
