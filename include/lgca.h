@@ -68,10 +68,11 @@ typedef struct lgca_info {
     uint64_t device_bytes;   /* everything the handle holds on the GPU        */
     uint64_t step;           /* absolute step counter (hash input)            */
     uint64_t seed;
-    int32_t steps_per_pass;  /* steps per launch of lgca_step: lgca_set_tuning's
-                                value, else the one the trial chose (2 or 4;
+    int32_t steps_per_pass;  /* steps per pass of lgca_step: lgca_set_tuning's
+                                value, else the one the trial chose (2, 4 or 8;
                                 -1 until it has)                                */
-    uint32_t reserved;
+    uint32_t pass_pipeline;  /* 1 when lgca_set_pipeline has turned on launches
+                                of several passes                               */
 } lgca_info;
 
 /* Creates a lattice of dim_x x dim_y cells on GPU `device`, all FLUID and
@@ -152,13 +153,30 @@ LGCA_API lgca_status lgca_step_noalloc(lgca_lattice* h, uint64_t nsteps);
 
 /* Launch shape of lgca_step: steps fused per pass over HBM (1, 2, 4 or 8; 0
  * selects the one-step kernel that keeps a 128-cell quad per thread; -1, the
- * default, is 4, or 2 where the trial finds 2-step passes faster on a lattice
- * too small to fill the GPU) and rows per tile (0 = automatic: a timed trial
+ * default, lets the trial choose: 4, 8 on a lattice that fills the GPU where
+ * 8-step passes are faster, or 2 where 2-step passes are faster on a lattice
+ * too small to fill it) and rows per tile (0 = automatic: a timed trial
  * of a few shapes on the first launches, kept thereafter and restarted by this
  * call, lgca_set_forcing, new cell types or a new stream).  Results do not
  * depend on it.  No counterpart in the reference; used by the autotuner and
  * benchmarks. */
 LGCA_API lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32_t tile_rows);
+/* Several passes of 4 or 8 steps per launch (on = 1; 0, the default, launches
+ * pass by pass).  Each XCD of the GPU takes a band of row tiles and runs its
+ * tiles pass after pass, each tile starting as soon as its neighbours have
+ * finished the previous pass, so one pass's last waves overlap the next
+ * pass's first instead of leaving the GPU to drain between launches.  Used
+ * once the launch shape is fixed (lgca_set_tuning) or kept (the trial, or
+ * lgca_tune), on lattices of full 32-cell words with at least two row tiles
+ * and 2K rows per XCD, as far as one window of force decisions reaches (256
+ * steps, fewer when the decisions of that many steps exceed 64 MiB); other
+ * steps run pass by pass.  Results do not depend on it.  lgca_synchronize
+ * and the state downloads report LGCA_ERR_HIP if a pass ever gave up waiting
+ * for its neighbours (a bounded wait; it cannot hang the GPU).  No
+ * counterpart in the reference.  In this build the mode is switched off:
+ * on = 1 fails with LGCA_ERR_INVALID_STATE (its waits did not complete on
+ * the GPU, for a reason not yet found), and on = 0 is accepted. */
+LGCA_API lgca_status lgca_set_pipeline(lgca_lattice* h, int on);
 /* Runs that trial now and waits for it: every candidate shape is timed on
  * passes that read the live state and write the other state buffer, and the
  * fastest per step is kept (lgca_get_info reports it).  The state and the

@@ -29,7 +29,7 @@ class Info(C.Structure):
         ("coarse_dim_y", C.c_uint32), ("words_per_row", C.c_uint32), ("row_pitch", C.c_uint32),
         ("tile_rows", C.c_uint32), ("num_cells", C.c_uint64), ("num_coarse_cells", C.c_uint64),
         ("state_bytes", C.c_uint64), ("device_bytes", C.c_uint64), ("step", C.c_uint64),
-        ("seed", C.c_uint64), ("steps_per_pass", C.c_int32), ("reserved", C.c_uint32),
+        ("seed", C.c_uint64), ("steps_per_pass", C.c_int32), ("pass_pipeline", C.c_uint32),
     ]
 
 
@@ -88,6 +88,7 @@ SIGNATURES = {
     "lgca_step_noalloc": [_h, C.c_uint64],
     "lgca_set_tuning": [_h, C.c_int, C.c_uint32],
     "lgca_tune": [_h],
+    "lgca_set_pipeline": [_h, C.c_int],
     "lgca_apply_body_force_count": [_h, C.c_int, C.c_int64, C.POINTER(C.c_int64)],
     "lgca_snapshot": [_h],
     "lgca_post_process": [_h, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],

@@ -46,6 +46,18 @@ struct WaveOrder {
     uint64_t key;
     uint32_t* dev;
 };
+// A pipelined launch's band plan (tile_pipe_plan) and its item order in
+// device memory, cached like WaveOrder.
+struct PipeCache {
+    int k;
+    uint32_t th, halo;
+    uint64_t key;
+    PipePlan plan;
+    uint32_t* order;
+};
+// Counter words of a pipelined launch: the bands' ticket counters, the error
+// word, then one finished-pass count per work item (row tile x strip).
+constexpr uint32_t kPipeQueue = 0, kPipeError = kMaxBands, kPipeDone = 16;
 // steps_per_pass value that lets the trial choose it
 constexpr int kAutoSteps = -1;
 }  // namespace lgca
@@ -66,6 +78,15 @@ struct lgca_lattice {
     std::vector<int32_t> row_lo, row_hi;  // host copy of row_range (empty without solids)
     uint64_t types_version = 0;     // counts cell-type uploads
     std::vector<WaveOrder> orders;  // cached dispatch orders
+    // several passes per launch (lgca_set_pipeline)
+    int pipe = 0;                   // 1: on where the launch shape allows it
+    uint32_t nxcc = 1, ncu = 0;     // XCDs and CUs of the device
+    std::vector<PipeCache> pipes;   // cached band plans
+    uint32_t* pipe_exch = nullptr;  // exchange rows
+    size_t pipe_exch_cap = 0;       // words
+    uint32_t* pipe_ctr = nullptr;   // counter words (kPipe*)
+    size_t pipe_ctr_cap = 0;        // words
+    bool pipe_check = false;        // a pipelined launch ran since its error word was read
     uint32_t* dec = nullptr;        // force decisions, [k][rows][fwn]
     size_t dec_cap = 0;             // words
     uint64_t dec_step0 = 0;
@@ -373,6 +394,13 @@ extern "C" lgca_status lgca_create(lgca_lattice** out, int model, uint32_t dim_x
     if ((e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(hip_fail(h, e, "hipStreamCreate"));
     h->stream = h->own_stream;
+    {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, device) == hipSuccess && n >= 1)
+            h->nxcc = (uint32_t)std::min<int>(n, (int)kMaxBands);
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n >= 1)
+            h->ncu = (uint32_t)n;
+    }
     const size_t sb = h->state_words * 4;
     if ((e = dalloc(&h->state[0], sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(state)"));
     if ((e = dalloc(&h->state[1], sb)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc(state)"));
@@ -402,6 +430,9 @@ extern "C" lgca_status lgca_destroy(lgca_lattice* h)
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         for (const WaveOrder& o : h->orders) (void)hipFree(o.dev);
+        for (const PipeCache& c : h->pipes) (void)hipFree(c.order);
+        if (h->pipe_exch) (void)hipFree(h->pipe_exch);
+        if (h->pipe_ctr) (void)hipFree(h->pipe_ctr);
         for (hipEvent_t ev : h->trial_ev) (void)hipEventDestroy(ev);
         if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     }
@@ -444,6 +475,7 @@ extern "C" lgca_status lgca_get_info(const lgca_lattice* h, lgca_info* info)
     info->seed = h->seed;
     if (h->tile_k != kAutoSteps) info->steps_per_pass = h->tile_k;
     else info->steps_per_pass = h->tile_th ? 4 : (h->tuned.th ? h->tuned.k : -1);
+    info->pass_pipeline = (uint32_t)h->pipe;
     return LGCA_OK;
 }
 
@@ -466,12 +498,25 @@ extern "C" lgca_status lgca_set_stream(lgca_lattice* h, void* hip_stream)
     return LGCA_OK;
 }
 
+// After a sync: the error word of the pipelined launches queued since the
+// last read (a pass that gave up waiting has left the state wrong).
+static lgca_status check_pipe(lgca_lattice* h)
+{
+    if (!h->pipe_check) return LGCA_OK;
+    h->pipe_check = false;
+    uint32_t e = 0;
+    HIPCHK(h, hipMemcpy(&e, h->pipe_ctr + kPipeError, 4, hipMemcpyDeviceToHost));
+    if (e) return fail(h, LGCA_ERR_HIP, "lgca_step: a pipelined pass gave up waiting for its neighbours");
+    return LGCA_OK;
+}
+
 extern "C" lgca_status lgca_synchronize(lgca_lattice* h)
 {
     if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    return LGCA_OK;
+    return check_pipe(h);
 }
 
 extern "C" lgca_status lgca_set_seed(lgca_lattice* h, uint64_t seed)
@@ -515,6 +560,8 @@ static lgca_status upload_types(lgca_lattice* h, const uint8_t* types, hipMemcpy
     ++h->types_version;
     for (const WaveOrder& o : h->orders) (void)hipFree(o.dev);
     h->orders.clear();
+    for (const PipeCache& c : h->pipes) (void)hipFree(c.order);
+    h->pipes.clear();
     h->row_lo.resize(h->dim_y);
     h->row_hi.resize(h->dim_y);
     HIPCHK(h, hipMemcpy(h->row_lo.data(), h->row_range, (size_t)h->dim_y * 4, hipMemcpyDeviceToHost));
@@ -586,14 +633,14 @@ static lgca_status download_state(lgca_lattice* h, uint8_t* dst, hipMemcpyKind k
     if (kind == hipMemcpyDeviceToDevice) {
         HIPCHK(h, launch_unpack(h->state[h->cur], dst, h->g, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
-        return LGCA_OK;
+        return check_pipe(h);
     }
     lgca_status st = ensure_bytes(h);
     if (st) return st;
     HIPCHK(h, launch_unpack(h->state[h->cur], h->bytes, h->g, h->stream));
     HIPCHK(h, hipMemcpyAsync(dst, h->bytes, n, kind, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    return LGCA_OK;
+    return check_pipe(h);
 }
 
 extern "C" lgca_status lgca_get_state_u8(lgca_lattice* h, uint8_t* host_state)
@@ -715,6 +762,14 @@ static lgca_status prepare_decisions(lgca_lattice* h, uint64_t remaining, uint32
     h->dec_step0 = h->step;
     h->dec_count = (uint32_t)k;
     return LGCA_OK;
+}
+
+// Steps of force decisions a pipelined launch may cover: up to 256, as far
+// as 64 MiB of decision words allow, at least one pass.
+static uint64_t pipe_window_steps(const lgca_lattice* h, int k)
+{
+    const uint64_t fit = (64ull << 20) / 4 / std::max<size_t>(decision_words_per_step(h), 1);
+    return std::max<uint64_t>(std::min<uint64_t>(fit, 256), (uint64_t)k);
 }
 
 // Rows per tile: tall enough to amortise the 2K halo rows, short enough to
@@ -875,6 +930,26 @@ extern "C" lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint
     return LGCA_OK;
 }
 
+// Multi-pass launches stay switched off: on the 4096 x 1024 periodic lattice
+// at 8 steps per pass and 16-row tiles, their waits did not complete (the
+// first launch ran past 100 s, with completion published first by a
+// read-modify-write count, then by device-coherent stores), and the cause is
+// not known.  lgca_set_pipeline(h, 1) fails until it is.
+constexpr bool kPipelineAvailable = false;
+
+extern "C" lgca_status lgca_set_pipeline(lgca_lattice* h, int on)
+{
+    if (!h) return LGCA_ERR_INVALID_ARGUMENT;
+    if (on != 0 && on != 1) return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_set_pipeline: on must be 0 or 1");
+    if (on && !kPipelineAvailable)
+        return fail(h, LGCA_ERR_INVALID_STATE,
+                    "lgca_set_pipeline: multi-pass launches are switched off in this build (their waits did "
+                    "not complete on the GPU)");
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->pipe = on;
+    return LGCA_OK;
+}
+
 // Arguments of one tiled pass from the live state into the other buffer, at
 // the current step, with rows per tile th and the force decisions of that
 // step (or null).
@@ -940,6 +1015,56 @@ static const uint32_t* wave_order(lgca_lattice* h, int k, const TileArgs& a, boo
     }
     h->orders.push_back(WaveOrder{k, a.th, a.halo, key, dev});
     return dev;
+}
+
+// The band plan of a pipelined launch of k-step passes with arguments a, built
+// and cached on first use (with the exchange rows and counters sized for
+// it); null when the shape does not allow one or an allocation fails (the
+// caller then launches pass by pass).
+static const PipeCache* pipe_plan(lgca_lattice* h, int k, const TileArgs& a)
+{
+    uint64_t key = h->types_version;
+    for (uint64_t v : {(uint64_t)(a.fdec != nullptr), (uint64_t)a.fw0, (uint64_t)a.fwn, (uint64_t)a.fy0,
+                       (uint64_t)a.fy1})
+        key = key * 0x100000001B3ull ^ v;
+    for (const PipeCache& c : h->pipes)
+        if (c.k == k && c.th == a.th && c.halo == a.halo && c.key == key) return &c;
+    PipeCache c{k, a.th, a.halo, key, PipePlan{}, nullptr};
+    const bool solids = h->mask && h->row_lo.size() == h->dim_y;
+    if (!tile_pipe_plan(a, k, h->nxcc, solids ? h->row_lo.data() : nullptr, solids ? h->row_hi.data() : nullptr,
+                        c.plan))
+        return nullptr;
+    const size_t xw = tile_pipe_exch_words(h->g, k, c.plan.nband);
+    const size_t cw = kPipeDone + (size_t)c.plan.ntiles * c.plan.nstrips;
+    // the buffers grow only between launches: wait for queued ones first
+    if ((xw > h->pipe_exch_cap || cw > h->pipe_ctr_cap || h->pipes.size() >= 16) &&
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return nullptr;
+    if (xw > h->pipe_exch_cap) {
+        if (h->pipe_exch) (void)hipFree(h->pipe_exch);
+        h->pipe_exch = nullptr;
+        h->pipe_exch_cap = 0;
+        if (dalloc(&h->pipe_exch, xw * 4) != hipSuccess) return nullptr;
+        h->pipe_exch_cap = xw;
+    }
+    if (cw > h->pipe_ctr_cap) {
+        if (h->pipe_ctr) (void)hipFree(h->pipe_ctr);
+        h->pipe_ctr = nullptr;
+        h->pipe_ctr_cap = 0;
+        if (dalloc(&h->pipe_ctr, cw * 4) != hipSuccess) return nullptr;
+        h->pipe_ctr_cap = cw;
+    }
+    if (dalloc(&c.order, c.plan.order.size() * 4) != hipSuccess) return nullptr;
+    if (hipMemcpy(c.order, c.plan.order.data(), c.plan.order.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(c.order);
+        return nullptr;
+    }
+    if (h->pipes.size() >= 16) {
+        (void)hipFree(h->pipes.front().order);
+        h->pipes.erase(h->pipes.begin());
+    }
+    h->pipes.push_back(std::move(c));
+    return &h->pipes.back();
 }
 
 static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_dec = nullptr,
@@ -1121,6 +1246,22 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_d
             lgca_status st = plan_launch(h, left, &k, &th, &ev_start, &ev_end);
             if (st) return st;
         }
+        // Several passes of the kept shape in one launch: all the steps left, as
+        // far as one window of force decisions reaches (not while the trial
+        // times single launches, nor without allocating)
+        uint64_t npass = 1;
+        const PipeCache* pc = nullptr;
+        if (h->pipe && !ext_dec && h->tile_k != 0 && ev_start < 0 && ev_end < 0 && h->trial_open < 0 &&
+            (k == 4 || k == 8) && left >= 2 * (uint64_t)k) {
+            uint64_t np = left / (uint64_t)k;
+            if (forcing) np = std::min<uint64_t>(np, pipe_window_steps(h, k) / (uint64_t)k);
+            if (np >= 2) {
+                // the plan reads only whether decisions exist, not where
+                pc = pipe_plan(h, k, with_k(tile_args(h, th, forcing ? h->state[0] : nullptr), k));
+                if (pc) npass = np;
+            }
+        }
+        const uint64_t span = npass * (uint64_t)k;  // steps of this launch
         const uint32_t* fdec = nullptr;
         if (forcing && ext_dec) {
             if (ext_count == 0 || h->step + k > ext_step0 + ext_count) {
@@ -1134,11 +1275,44 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_d
             fdec = ext_dec + (size_t)(h->step - ext_step0) * per_step;
         } else if (forcing) {
             if (h->dec_count == 0 || h->step < h->dec_step0 ||
-                h->step + k > h->dec_step0 + h->dec_count) {
-                lgca_status st = prepare_decisions(h, left, (uint32_t)k);
+                h->step + span > h->dec_step0 + h->dec_count) {
+                lgca_status st = prepare_decisions(h, left, (uint32_t)span);
                 if (st) return st;
             }
             fdec = h->dec + (size_t)(h->step - h->dec_step0) * per_step;
+        }
+        if (pc) {
+            const PipePlan& pl = pc->plan;
+            const TileArgs a = with_k(tile_args(h, th, fdec), k);
+            PipeArgs p{};
+            p.buf[0] = h->state[h->cur];
+            p.buf[1] = h->state[h->cur ^ 1];
+            p.npass = (uint32_t)npass;
+            p.nband = pl.nband;
+            p.nstrips = pl.nstrips;
+            p.ntiles = pl.ntiles;
+            p.queue = h->pipe_ctr + kPipeQueue;
+            p.error = h->pipe_ctr + kPipeError;
+            p.done = h->pipe_ctr + kPipeDone;
+            p.order = pc->order;
+            for (uint32_t b = 0; b < pl.nband; ++b) {
+                p.band_first[b] = pl.band_first[b];
+                p.band_items[b] = pl.band_items[b];
+                p.band_y0[b] = pl.band_y0[b];
+                p.band_y1[b] = pl.band_y1[b];
+            }
+            p.exch = h->pipe_exch;
+            p.fdec_pass = (uint64_t)k * per_step;
+            // every wave slot of the GPU (two waves per SIMD at 8 steps, four at 4)
+            uint32_t nwaves = std::max<uint32_t>(h->ncu, 1) * 4u * (k == 8 ? 2u : 4u);
+            nwaves = (nwaves + pl.nband - 1) / pl.nband * pl.nband;
+            HIPCHK(h, launch_fill_u32(h->pipe_ctr, 0u, kPipeDone + (size_t)pl.ntiles * pl.nstrips, h->stream));
+            HIPCHK(h, launch_tile_pipe(h->kind, k, a, p, nwaves, h->stream));
+            h->pipe_check = true;
+            h->cur ^= (int)(npass & 1u);
+            h->step += span;
+            left -= span;
+            continue;
         }
         if (h->tile_k == 0) {
             StepArgs a{};

@@ -87,6 +87,43 @@ struct TileOrder {
     const uint32_t* order;      // device order (tile_wave_order), or null
 };
 
+// Several K-step passes in one launch (tile_pipe_kernel): one row band per
+// XCD, each band's work items handed out pass after pass, each item waiting
+// for the previous pass of the tiles it reads.
+constexpr uint32_t kMaxBands = 8;
+struct PipeArgs {
+    uint32_t* buf[2];            // pass p reads buf[p & 1] and writes buf[(p + 1) & 1]
+    uint32_t npass;              // passes in the launch
+    uint32_t nband;              // row bands, one per XCD
+    uint32_t nstrips, ntiles;
+    uint32_t* queue;             // [nband] tickets handed out (0 at launch)
+    uint32_t* done;              // [ntiles][nstrips] passes each item has finished (0 at launch)
+    uint32_t* error;             // set when a wait gives up (0 at launch)
+    const uint32_t* order;       // the bands' items (tile * nstrips + strip), band after band
+    uint32_t band_first[kMaxBands], band_items[kMaxBands];
+    uint32_t band_y0[kMaxBands], band_y1[kMaxBands];
+    uint32_t* exch;              // [2][nband][2][K][Q][P] words: each band's first and last K rows
+    uint64_t fdec_pass;          // decision words per pass (K steps)
+};
+// Band partition and item order of a pipelined launch: nband bands of whole
+// row tiles with about equal work (walks over solids counted twice), items
+// tile-major within each band.  False when the shape does not allow one
+// (segment mode, a short last word, K not 4 or 8, th < K, fewer than two
+// tiles or 2K rows per band).
+struct PipePlan {
+    std::vector<uint32_t> order;
+    uint32_t nband = 0, nstrips = 0, ntiles = 0;
+    uint32_t band_first[kMaxBands] = {}, band_items[kMaxBands] = {};
+    uint32_t band_y0[kMaxBands] = {}, band_y1[kMaxBands] = {};
+};
+bool tile_pipe_plan(const TileArgs& a, int k, uint32_t nband, const int32_t* row_lo, const int32_t* row_hi,
+                    PipePlan& plan);
+size_t tile_pipe_exch_words(const Geometry& g, int k, uint32_t nband);
+// Launches p.npass passes of k steps from a.t0 (a.src / a.dst unused; p.buf
+// holds the buffers) on nwaves waves; p's counters must be zero.
+hipError_t launch_tile_pipe(int kind, int k, const TileArgs& a, const PipeArgs& p, uint32_t nwaves,
+                            hipStream_t s);
+
 // Batched ensembles.  BatchJob is a job's static descriptor (its TileArgs
 // without src/dst/t0/th/halo, which come from BatchState and WorkItem);
 // BatchState is the job's dynamic state on the device; a WorkItem is one
@@ -126,6 +163,8 @@ hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves
 template <int K>
 hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
                                 uint32_t n_items, uint32_t n_heavy, bool ragged, hipStream_t s);
+template <int KIND, int K>
+hipError_t launch_tile_pipe_kk(const TileArgs& a, const PipeArgs& p, uint32_t nwaves, hipStream_t s);
 
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]

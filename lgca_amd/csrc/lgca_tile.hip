@@ -68,25 +68,24 @@ void append_xcd_runs(std::vector<uint32_t>& order, const std::vector<uint32_t>& 
     }
 }
 
-}  // namespace
-
-bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int32_t* row_hi,
-                     std::vector<uint32_t>& order)
+// The cost class of every strip-mode wave of a pass (tile * nstrips + strip):
+// 2 when its walk crosses solids (about twice a plain walk), 1 when it crosses
+// forced words (about a sixth longer), else 0.  False when the pass is not in
+// strip mode or has too many waves to list.
+bool wave_classes(const TileArgs& a, int k, const int32_t* row_lo, const int32_t* row_hi,
+                  uint32_t& nstrips, uint32_t& ntiles, std::vector<uint8_t>& cls)
 {
-    order.clear();
     if (tile_segments(a.g, k) || a.th < 2) return false;
     const uint32_t stride = 64u - 2u * a.halo;
-    const uint32_t nstrips = (a.g.W + stride - 1) / stride;
-    const uint32_t ntiles = (a.g.dim_y + a.th - 1) / a.th;
+    nstrips = (a.g.W + stride - 1) / stride;
+    ntiles = (a.g.dim_y + a.th - 1) / a.th;
     if ((uint64_t)nstrips * ntiles > ((uint64_t)1 << 24)) return false;
     const bool forced = a.fdec && a.fwn;
     std::vector<uint8_t> fstrip(nstrips, 0);
     if (forced)
         for (uint32_t st = 0; st < nstrips; ++st)
             fstrip[st] = strip_hits(a.g, stride, a.halo, st, a.fw0, (int64_t)a.fw0 + a.fwn);
-    // the cost classes: walks over solids (about twice a plain walk), over
-    // forced words (about a sixth more), plain
-    std::vector<uint32_t> solid, force, plain;
+    cls.assign((size_t)nstrips * ntiles, 0);
     const int64_t dy = a.g.dim_y;
     for (uint32_t t = 0; t < ntiles; ++t) {
         const int64_t y0 = (int64_t)t * a.th, rows = std::min<int64_t>(a.th, dy - y0);
@@ -101,12 +100,27 @@ bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int3
             frow |= r >= (int64_t)a.fy0 && r < (int64_t)a.fy1;
         }
         for (uint32_t st = 0; st < nstrips; ++st) {
-            const uint32_t it = t * nstrips + st;
-            if (lo < hi && strip_hits(a.g, stride, a.halo, st, lo, hi)) solid.push_back(it);
-            else if (forced && frow && fstrip[st]) force.push_back(it);
-            else plain.push_back(it);
+            uint8_t c = 0;
+            if (lo < hi && strip_hits(a.g, stride, a.halo, st, lo, hi)) c = 2;
+            else if (forced && frow && fstrip[st]) c = 1;
+            cls[(size_t)t * nstrips + st] = c;
         }
     }
+    return true;
+}
+
+}  // namespace
+
+bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int32_t* row_hi,
+                     std::vector<uint32_t>& order)
+{
+    order.clear();
+    uint32_t nstrips = 0, ntiles = 0;
+    std::vector<uint8_t> cls;
+    if (!wave_classes(a, k, row_lo, row_hi, nstrips, ntiles, cls)) return false;
+    std::vector<uint32_t> solid, force, plain;
+    for (uint32_t it = 0; it < (uint32_t)cls.size(); ++it)
+        (cls[it] == 2 ? solid : cls[it] == 1 ? force : plain).push_back(it);
     // The few long solid walks go first, spread over the XCDs; then the
     // forced and the plain ones, each in contiguous runs per XCD, so that
     // the tiles an XCD holds are vertical neighbours whose halo rows meet in
@@ -116,6 +130,67 @@ bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int3
     append_xcd_runs(order, force);
     append_xcd_runs(order, plain);
     return true;
+}
+
+bool tile_pipe_plan(const TileArgs& a, int k, uint32_t nband, const int32_t* row_lo, const int32_t* row_hi,
+                    PipePlan& plan)
+{
+    plan = PipePlan{};
+    if ((k != 4 && k != 8) || nband < 1 || nband > kMaxBands) return false;
+    if (a.g.nb != 32 || a.th < (uint32_t)k || (a.th & 1u)) return false;
+    uint32_t nstrips = 0, ntiles = 0;
+    std::vector<uint8_t> cls;
+    if (!wave_classes(a, k, row_lo, row_hi, nstrips, ntiles, cls)) return false;
+    if (ntiles < 2 * nband) return false;
+    // bands of whole tiles with about equal work, at least two tiles each
+    std::vector<double> cost(ntiles + 1, 0.0);  // prefix sums
+    for (uint32_t t = 0; t < ntiles; ++t) {
+        double c = 0.0;
+        for (uint32_t st = 0; st < nstrips; ++st) c += 1.0 + (double)cls[(size_t)t * nstrips + st] * 0.5;
+        cost[t + 1] = cost[t] + c;
+    }
+    std::vector<uint32_t> cut(nband + 1, 0);
+    cut[nband] = ntiles;
+    for (uint32_t b = 1; b < nband; ++b) {
+        const double target = cost[ntiles] * b / nband;
+        uint32_t t = cut[b - 1] + 2;
+        while (t < ntiles && cost[t] + 0.5 * (cost[t + 1] - cost[t]) < target) ++t;
+        cut[b] = std::min(std::max(t, cut[b - 1] + 2), ntiles - 2 * (nband - b));
+    }
+    for (uint32_t b = 0; b < nband; ++b) {
+        const uint32_t y0 = cut[b] * a.th, y1 = std::min<uint64_t>((uint64_t)cut[b + 1] * a.th, a.g.dim_y);
+        if (y1 - y0 < 2u * (uint32_t)k) return false;
+        plan.band_y0[b] = y0;
+        plan.band_y1[b] = y1;
+        plan.band_first[b] = (uint32_t)plan.order.size();
+        for (uint32_t t = cut[b]; t < cut[b + 1]; ++t)
+            for (uint32_t st = 0; st < nstrips; ++st) plan.order.push_back(t * nstrips + st);
+        plan.band_items[b] = (uint32_t)plan.order.size() - plan.band_first[b];
+    }
+    plan.nband = nband;
+    plan.nstrips = nstrips;
+    plan.ntiles = ntiles;
+    return true;
+}
+
+size_t tile_pipe_exch_words(const Geometry& g, int k, uint32_t nband)
+{
+    return (size_t)2 * nband * 2 * (size_t)k * g.Q * g.P;
+}
+
+hipError_t launch_tile_pipe(int kind, int k, const TileArgs& a, const PipeArgs& p, uint32_t nwaves, hipStream_t s)
+{
+    if (tile_segments(a.g, k) || a.g.nb != 32 || a.th < (uint32_t)k || (a.th & 1u) || a.th > 65535)
+        return hipErrorInvalidValue;
+    if (a.halo < tile_halo(a.g, k) || a.halo > 16 || p.nband < 1 || p.nband > kMaxBands || !nwaves)
+        return hipErrorInvalidValue;
+#define LGCA_PIPE_CASE(KD, KK) \
+    if (kind == KD && k == KK) return launch_tile_pipe_kk<KD, KK>(a, p, nwaves, s);
+    LGCA_PIPE_CASE(kHPP, 4) LGCA_PIPE_CASE(kHPP, 8)
+    LGCA_PIPE_CASE(kFHP1, 4) LGCA_PIPE_CASE(kFHP1, 8)
+    LGCA_PIPE_CASE(kFHP2, 4) LGCA_PIPE_CASE(kFHP2, 8)
+#undef LGCA_PIPE_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order)

@@ -79,6 +79,18 @@ __device__ __forceinline__ void bstore(uint32_t v, __amdgpu_buffer_rsrc_t r, uin
 {
     __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, __builtin_amdgcn_readfirstlane(soff), 0);
 }
+// The same at system coherence (cache policy sc0 sc1): the store is written
+// through to memory and the load reads the coherent copy, so rows that one
+// XCD writes and another reads in the same launch need no L2 maintenance.
+constexpr int kCoherent = 17;
+__device__ __forceinline__ uint32_t bload_coherent(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, __builtin_amdgcn_readfirstlane(soff), kCoherent);
+}
+__device__ __forceinline__ void bstore_coherent(uint32_t v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, __builtin_amdgcn_readfirstlane(soff), kCoherent);
+}
 
 // Neighbouring words: the adjacent lane (DPP) when a wave holds consecutive
 // words of one row, or the lane holding word w -+ 1 of the same row in
@@ -155,8 +167,22 @@ template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 
 // point of their walks (a down tile ends where the up tile below it ends, and
 // starts where the up tile above it starts), so the second read of each halo
 // row finds it in the L2 cache of their XCD instead of HBM.
-template <int KIND, int K, int O, bool RAGGED, bool SEG, bool ALT = false>
-__device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile)
+// PIPE (tile_pipe_kernel, strip mode): the pass is one of several in a
+// launch, and the wave's XCD owns the rows [pw.y0, pw.y1) of the lattice.
+// Rows of the neighbouring bands (at most K above or below the band) come
+// from their owners' exchange rows of the previous pass, after the first
+// pass of the launch; the band's own first and last K output rows also go to
+// its exchange rows for this pass.
+struct PipeWalk {
+    uint32_t y0, y1;        // the band's rows
+    uint32_t pass;          // pass within the launch
+    uint32_t band, prev, next, nband;
+    uint32_t* exch;         // [pass & 1][band][0 first / 1 last][K rows][Q planes][P words]
+};
+
+template <int KIND, int K, int O, bool RAGGED, bool SEG, bool ALT = false, bool PIPE = false>
+__device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile,
+                                          const PipeWalk* pw = nullptr)
 {
     constexpr int Q = num_dir_of_kind(KIND);
     const Geometry& g = a.g;
@@ -211,8 +237,24 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     // Segment mode: rows differ per lane, so one descriptor spans the buffer
     // (W <= 64 keeps it under 4 GiB) and the row goes into the lane offset.
     const uint32_t plane_b = g.P * 4u, row_b = (uint32_t)Q * plane_b;
+    // PIPE: exchange row k of side `side` (0: the band's first K rows, 1: its
+    // last K) of band b, for the pass of parity par
+    auto exch_row = [&](uint32_t par, uint32_t b, uint32_t side, uint32_t k) -> uint32_t* {
+        return pw->exch + (size_t)((((par * pw->nband + b) * 2u + side) * K + k) * Q) * g.P;
+    };
     auto load_plane = [&](uint32_t r, int d) -> uint32_t {
         if (SEG) return bload(make_rsrc(src, g.dim_y * row_b), r * row_b + c.wb, (uint32_t)d * plane_b);
+        if (PIPE && pw->pass > 0 && r - pw->y0 >= pw->y1 - pw->y0) {
+            // a neighbouring band's row: t rows above the band, or u below it
+            uint32_t t = pw->y0 + g.dim_y - r;
+            if (t >= g.dim_y) t -= g.dim_y;
+            uint32_t u = r + g.dim_y - pw->y1;
+            if (u >= g.dim_y) u -= g.dim_y;
+            const uint32_t par = (pw->pass - 1u) & 1u;
+            const uint32_t* row = (t >= 1 && t <= (uint32_t)K) ? exch_row(par, pw->prev, 1, K - t)
+                                                               : exch_row(par, pw->next, 0, u);
+            return bload_coherent(make_rsrc(row, row_b), c.wb, (uint32_t)d * plane_b);
+        }
         return bload(make_rsrc(src + (size_t)r * Q * g.P, row_b), c.wb, (uint32_t)d * plane_b);
     };
     // Stores and force-decision loads are issued by every lane: a lane that
@@ -226,6 +268,15 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     auto store_plane = [&](uint32_t r, int d, uint32_t v, bool wr) {
         if (SEG) bstore(v, make_rsrc(dst, g.dim_y * row_b), wr ? r * row_b + c.wb : kOutside, (uint32_t)d * plane_b);
         else bstore(v, make_rsrc(dst + (size_t)r * Q * g.P, row_b), wr ? c.wb : kOutside, (uint32_t)d * plane_b);
+        if (PIPE) {  // the band's first and last K rows also go to its exchange rows
+            const uint32_t par = pw->pass & 1u, o0 = r - pw->y0, o1 = pw->y1 - 1u - r;
+            if (o0 < (uint32_t)K)
+                bstore_coherent(v, make_rsrc(exch_row(par, pw->band, 0, o0), row_b), wr ? c.wb : kOutside,
+                                (uint32_t)d * plane_b);
+            if (o1 < (uint32_t)K)
+                bstore_coherent(v, make_rsrc(exch_row(par, pw->band, 1, K - 1u - o1), row_b),
+                                wr ? c.wb : kOutside, (uint32_t)d * plane_b);
+        }
     };
     auto load_mask = [&](uint32_t r, int b) -> uint32_t {
         if (SEG) return bload(make_rsrc(a.mask, g.dim_y * 2u * plane_b), r * 2u * plane_b + c.wb, (uint32_t)b * plane_b);
@@ -635,6 +686,106 @@ void tile_kernel(TileArgs a, TileOrder o)
     tile_body<KIND, K, O, RAGGED, false, true>(a, strip, tile);
 }
 
+// Several passes in one launch (strip mode, K = 4 or 8, th >= K), so that the
+// end of one pass overlaps the start of the next instead of leaving the GPU
+// to drain between launches.  Each XCD owns a band of whole row tiles
+// (p.band_y0 / band_y1, its own L2 holding the band's rows) and hands out the
+// band's work items pass after pass from its own ticket counter.  A wave
+// takes a ticket, waits until every strip of every tile its walk reads (rows
+// [y0 - K, y0 + rows + K)) has finished the previous pass, and runs the pass.
+// Tickets go out in pass order, so every item a wave waits for is held by a
+// running wave, which waits only for earlier passes: the waits always end.
+// They are bounded anyway (kPipeSpinLimit polls): a wave that gives up sets
+// p.error and leaves, and the host reports the error.
+//
+// Memory: an item publishes that it has finished pass j by storing j + 1 into
+// its own word of p.done with a device-coherent store, after all its other
+// stores have completed, and waiters poll those words with device-coherent
+// loads, 64 strips per load: flags that one wave stores and others poll,
+// both bypassing the non-coherent caches, as the look-back scans of rocPRIM
+// do on these GPUs (a read-modify-write count leaves no such guarantee for
+// the polls of other XCDs).  Rows written by other CUs of the same XCD meet
+// in its L2, so after its waits a wave drops its CU's L1 copies
+// (buffer_inv sc0) and reads them as usual.  Rows of the neighbouring bands,
+// written on other XCDs, go through the exchange rows with system-coherent
+// stores and loads (bload_coherent); the first pass of a launch reads them
+// from the state buffer, which the previous launch wrote.  The state buffers
+// are written and read by one XCD each during the launch, and each band's
+// ticket counter by the waves of one XCD.
+constexpr uint32_t kPipeSpinLimit = 1u << 20;
+constexpr uint32_t kXccIdReg = (31u << 11) | 20u;  // hwreg(HW_REG_XCC_ID), all 32 bits
+
+// Waits until every strip of tile u has finished `need` passes; lane l polls
+// strip s0 + l.
+__device__ __forceinline__ bool pipe_wait(const PipeArgs& p, uint32_t u, uint32_t need, uint32_t lane)
+{
+    for (uint32_t s0 = 0; s0 < p.nstrips; s0 += 64) {
+        const bool mine = s0 + lane < p.nstrips;
+        const uint32_t* w = p.done + (size_t)u * p.nstrips + s0 + (mine ? lane : 0u);
+        for (uint32_t spins = 0;; ++spins) {
+            const uint32_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__builtin_amdgcn_ballot_w64(mine && v < need) == 0) break;
+            if (spins >= kPipeSpinLimit) {
+                __hip_atomic_store(p.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+    }
+    return true;
+}
+
+template <int KIND, int K, int O>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
+void tile_pipe_kernel(TileArgs a, PipeArgs p)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t band = (__builtin_amdgcn_s_getreg(kXccIdReg) & 15u) % p.nband;
+    PipeWalk pw;
+    pw.y0 = p.band_y0[band];
+    pw.y1 = p.band_y1[band];
+    pw.band = band;
+    pw.prev = band == 0 ? p.nband - 1u : band - 1u;
+    pw.next = band + 1u == p.nband ? 0u : band + 1u;
+    pw.nband = p.nband;
+    pw.exch = p.exch;
+    const uint32_t nb = p.band_items[band], dy = a.g.dim_y;
+    for (;;) {
+        uint32_t tk = 0;
+        if (lane == 0) tk = __hip_atomic_fetch_add(p.queue + band, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tk = __builtin_amdgcn_readfirstlane(tk);
+        if (tk >= nb * p.npass) return;
+        const uint32_t pass = tk / nb, it = p.order[p.band_first[band] + (tk - pass * nb)];
+        const uint32_t tile = it / p.nstrips, strip = it - tile * p.nstrips;
+        if (pass > 0) {
+            // the tiles holding rows [y0 - K, y0 + rows + K), cyclically
+            const uint32_t y0 = tile * a.th, rows = min(a.th, dy - y0);
+            const uint32_t lo = y0 >= (uint32_t)K ? y0 - K : y0 + dy - K;
+            uint32_t hi = y0 + rows + K - 1u;
+            if (hi >= dy) hi -= dy;
+            const uint32_t first = lo / a.th, last = hi / a.th;
+            for (uint32_t u = first, n = 0;; u = u + 1u == p.ntiles ? 0u : u + 1u, ++n) {
+                if (!pipe_wait(p, u, pass, lane)) return;
+                if (u == last || n >= 4) break;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            asm volatile("buffer_inv sc0" ::: "memory");
+        }
+        TileArgs b = a;
+        b.src = p.buf[pass & 1u];
+        b.dst = p.buf[(pass + 1u) & 1u];
+        b.t0 = a.t0 + pass * (uint32_t)K;
+        if (a.fdec) b.fdec = a.fdec + (size_t)pass * p.fdec_pass;
+        pw.pass = pass;
+        tile_body<KIND, K, O, false, false, true, true>(b, strip, tile, &pw);
+        // the pass's stores have all completed before the item says so
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (lane == 0)
+            __hip_atomic_store(p.done + (size_t)tile * p.nstrips + strip, pass + 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Batched form: every wave takes one work item (job, strip, tile, shape) and
 // runs that job's lattice.  A job's step, current buffer and force-window
 // origin live in device memory (the job's last wave of a round moves them on),
@@ -728,6 +879,21 @@ hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves
 #undef LGCA_TILE_OFF
 #undef LGCA_TILE_LAUNCH
     return hipErrorInvalidValue;
+}
+
+// Pipelined launcher (K = 4 or 8, no short last word); lgca_tile.hip has
+// built the plan and checked the shape.
+template <int KIND, int K>
+hipError_t launch_tile_pipe_kk(const TileArgs& a, const PipeArgs& p, uint32_t nwaves, hipStream_t s)
+{
+    static_assert(K == 4 || K == 8, "pipelined passes keep the chirality offset: K = 4 or 8");
+    switch (a.t0 & 3u) {
+    case 0: hipLaunchKernelGGL((tile_pipe_kernel<KIND, K, 0>), dim3(nwaves), dim3(64), 0, s, a, p); break;
+    case 1: hipLaunchKernelGGL((tile_pipe_kernel<KIND, K, 1>), dim3(nwaves), dim3(64), 0, s, a, p); break;
+    case 2: hipLaunchKernelGGL((tile_pipe_kernel<KIND, K, 2>), dim3(nwaves), dim3(64), 0, s, a, p); break;
+    default: hipLaunchKernelGGL((tile_pipe_kernel<KIND, K, 3>), dim3(nwaves), dim3(64), 0, s, a, p); break;
+    }
+    return hipGetLastError();
 }
 
 template <int K>

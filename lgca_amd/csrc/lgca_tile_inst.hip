@@ -14,6 +14,10 @@ namespace lgca {
 #if LGCA_INST_KIND >= 0
 template hipError_t launch_tile_kk<LGCA_INST_KIND, LGCA_INST_K>(const TileArgs&, const TileOrder&, uint32_t,
                                                                 bool, bool, hipStream_t);
+#if LGCA_INST_K >= 4
+template hipError_t launch_tile_pipe_kk<LGCA_INST_KIND, LGCA_INST_K>(const TileArgs&, const PipeArgs&, uint32_t,
+                                                                     hipStream_t);
+#endif
 #else
 template hipError_t launch_tile_batch_kk<LGCA_INST_K>(const BatchJob*, BatchState*, const WorkItem*, uint32_t,
                                                       uint32_t, bool, hipStream_t);

@@ -278,6 +278,8 @@ public:
         sync_types();
         check(lgca_tune(dev()));
     }
+    // Several passes per launch of lgca_step (lgca_set_pipeline).
+    void set_pipeline(bool on) { check(lgca_set_pipeline(dev(), on ? 1 : 0)); }
     // The viewer's forcing controller on the GPU: a tick decides on the
     // snapshot's mean velocity, applies the count-mode force, runs `steps`
     // steps and snapshots, without the host waiting (lgca_controller_*).

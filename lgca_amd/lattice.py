@@ -242,6 +242,11 @@ class Lattice:
         (0 = automatic)."""
         self._check(lib().lgca_set_tuning(self._h, int(steps_per_pass), int(tile_rows)))
 
+    def set_pipeline(self, on: bool) -> None:
+        """Several passes per launch of step() (each XCD runs a band of row
+        tiles pass after pass); see lgca_set_pipeline."""
+        self._check(lib().lgca_set_pipeline(self._h, 1 if on else 0))
+
     def tune(self) -> None:
         """Times the candidate launch shapes now (without changing the state or
         the step) and keeps the fastest; see lgca_tune."""

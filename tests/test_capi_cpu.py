@@ -122,3 +122,4 @@ def test_ctypes_structures_match_the_c_layout(tmp_path):
         py = structs[cname]
         want = C.sizeof(py) if field == "size" else getattr(py, field).offset
         assert int(value) == want, line
+
