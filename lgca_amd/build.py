@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblgca_hip.so")
 
-SOURCES = ["lgca_kernels.hip", "lgca_tile.hip", "lgca_capi.hip", "lgca_batch.hip"]
+SOURCES = ["lgca_kernels.hip", "lgca_tile.hip", "lgca_capi.hip", "lgca_batch.hip", "lgca_plan.cpp"]
 # lgca_tile_inst.hip is compiled once per (model kind, steps per pass) pair of
 # the single-lattice kernels and once per steps per pass of the batch kernel
 # (kind -1), which lets the template-heavy tiled kernels compile in parallel

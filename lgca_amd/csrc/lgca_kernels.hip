@@ -14,22 +14,6 @@
 
 namespace lgca {
 
-Geometry make_geometry(uint32_t dim_x, uint32_t dim_y, uint32_t Q)
-{
-    Geometry g{};
-    g.dim_x = dim_x;
-    g.dim_y = dim_y;
-    g.W = (dim_x + 31) / 32;
-    g.P = (g.W + 3) & ~3u;
-    g.Wq = g.P / 4;
-    g.nb = dim_x - 32 * (g.W - 1);
-    g.Q = Q;
-    uint32_t l = 0;
-    while ((1u << l) < g.Wq && l < 6) ++l;
-    g.qb_log = l;
-    return g;
-}
-
 static inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 __device__ __forceinline__ uint32_t valid_mask(const Geometry& g, uint32_t w)

@@ -64,6 +64,9 @@ uint32_t tile_strips(const Geometry& g, int k);  // x-strips per row tile in str
 // Segment mode: rows of W <= 64 words, S = 64 / W whole rows per wave (one per
 // row tile), used when it beats strips (S >= 2, or a row needing two strips).
 bool tile_segments(const Geometry& g, int k);
+// Does strip st of a pass (words [st * stride - halo, + 64) of its rows, taken
+// modulo W) touch words [lo, hi) of those rows?
+bool tile_strip_hits(const Geometry& g, uint32_t stride, uint32_t halo, uint32_t st, int64_t lo, int64_t hi);
 // A pass of k steps; `order` is a dispatch order from tile_wave_order for the
 // same arguments and k, in device memory, or null.
 hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order = nullptr);
