@@ -45,6 +45,13 @@ struct WaveOrder {
     uint32_t th, halo;
     uint64_t key;
     uint32_t* dev;
+    uint32_t n;  // entries
+};
+// A dispatch order for launch_tile: entries in device memory and their count
+// (null and 0: the kernel's own order).
+struct OrderRef {
+    const uint32_t* dev = nullptr;
+    uint32_t n = 0;
 };
 // A pipelined launch's band plan (tile_pipe_plan) and its item order in
 // device memory, cached like WaveOrder.
@@ -824,6 +831,40 @@ static int steps_for_launch(int want, uint64_t left)
 constexpr uint32_t kTrialReps = 2;
 
 // The trial's candidate shapes, the rule's own first.
+static TileArgs tile_args(const lgca_lattice* h, uint32_t th, const uint32_t* fdec);
+static TileArgs with_k(TileArgs a, int k);
+
+// Waves of a strip-mode pass of k steps and th-row tiles (tile_wave_order's
+// entries: solid walks count once per part), 0 when it has no such order.
+static uint64_t pass_waves(const lgca_lattice* h, int k, uint32_t th)
+{
+    static const uint32_t any_decision = 0;  // only its presence is read
+    const TileArgs a = with_k(tile_args(h, th, h->force_thr ? &any_decision : nullptr), k);
+    const bool solids = h->mask && h->row_lo.size() == h->dim_y;
+    std::vector<uint32_t> v;
+    if (!tile_wave_order(a, k, solids ? h->row_lo.data() : nullptr, solids ? h->row_hi.data() : nullptr, v))
+        return 0;
+    return v.size();
+}
+
+// The shortest even tile height (8 rows or more) whose k-step passes have at
+// most `slots` waves, or 0 when none has.  The wave count falls as tiles grow
+// (fewer tiles, each cut the same way), so a bisection finds it.
+static uint32_t fill_tile_rows(const lgca_lattice* h, int k, uint64_t slots)
+{
+    uint32_t lo = 4, hi = std::min<uint32_t>(h->dim_y, 65534) / 2;  // heights 2 lo .. 2 hi
+    if (hi < lo) return 0;
+    const uint64_t top = pass_waves(h, k, 2 * hi);
+    if (!top || top > slots) return 0;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint64_t n = pass_waves(h, k, 2 * mid);
+        if (n && n <= slots) hi = mid;
+        else lo = mid + 1;
+    }
+    return 2 * lo;
+}
+
 static std::vector<LaunchShape> trial_candidates(const lgca_lattice* h)
 {
     const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
@@ -839,8 +880,22 @@ static std::vector<LaunchShape> trial_candidates(const lgca_lattice* h)
         add(2, 2);
         add(2, 4);
     }
-    if (h->tile_k == kAutoSteps && c[0].th >= 4)
+    if (h->tile_k == kAutoSteps && c[0].th >= 4) {
         for (uint32_t r : {16u, 24u, 32u, 40u, 48u, 64u}) add(8, r);
+        // Tiles tall enough that a pass's waves fill the GPU's wave slots
+        // once (or twice) and no more: a pass with a few waves past a whole
+        // number of rounds ends with a tail of them on an otherwise idle GPU
+        // (32768 x 8192 Karman lattice, 8-step passes, 2048 slots: 76-row
+        // tiles, 1986 waves, 21.1 us per step; 72 rows, 2094 waves, 23.2 us)
+        const uint32_t slots = h->ncu * (uint32_t)std::max(tile_waves_per_cu(h->kind, 8, h->g.nb != 32), 0);
+        for (uint32_t m = 1; m <= 2 && slots && !tile_segments(h->g, 8); ++m) {
+            const uint32_t th = fill_tile_rows(h, 8, (uint64_t)m * slots);
+            if (th) {
+                add(8, th);
+                add(8, th + 4);
+            }
+        }
+    }
     return c;
 }
 
@@ -987,34 +1042,34 @@ static TileArgs with_k(TileArgs a, int k)
 // and cached on first use; null in segment mode, when it cannot be built, or
 // when it is not cached and may_alloc is false (the kernel then uses its own
 // fixed order; results never depend on the order).
-static const uint32_t* wave_order(lgca_lattice* h, int k, const TileArgs& a, bool may_alloc)
+static OrderRef wave_order(lgca_lattice* h, int k, const TileArgs& a, bool may_alloc)
 {
-    if (tile_segments(h->g, k)) return nullptr;
+    if (tile_segments(h->g, k)) return {};
     uint64_t key = h->types_version;
     for (uint64_t v : {(uint64_t)(a.fdec != nullptr), (uint64_t)a.fw0, (uint64_t)a.fwn, (uint64_t)a.fy0,
                        (uint64_t)a.fy1})
         key = key * 0x100000001B3ull ^ v;
     for (const WaveOrder& o : h->orders)
-        if (o.k == k && o.th == a.th && o.halo == a.halo && o.key == key) return o.dev;
-    if (!may_alloc) return nullptr;
+        if (o.k == k && o.th == a.th && o.halo == a.halo && o.key == key) return {o.dev, o.n};
+    if (!may_alloc) return {};
     std::vector<uint32_t> v;
     const bool solids = h->mask && h->row_lo.size() == h->dim_y;
     if (!tile_wave_order(a, k, solids ? h->row_lo.data() : nullptr, solids ? h->row_hi.data() : nullptr, v))
-        return nullptr;
+        return {};
     if (h->orders.size() >= 16) {
         // the oldest may still be read by queued passes
-        if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return {};
         (void)hipFree(h->orders.front().dev);
         h->orders.erase(h->orders.begin());
     }
     uint32_t* dev = nullptr;
-    if (dalloc(&dev, v.size() * 4) != hipSuccess) return nullptr;
+    if (dalloc(&dev, v.size() * 4) != hipSuccess) return {};
     if (hipMemcpy(dev, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(dev);
-        return nullptr;
+        return {};
     }
-    h->orders.push_back(WaveOrder{k, a.th, a.halo, key, dev});
-    return dev;
+    h->orders.push_back(WaveOrder{k, a.th, a.halo, key, dev, (uint32_t)v.size()});
+    return {dev, (uint32_t)v.size()};
 }
 
 // The band plan of a pipelined launch of k-step passes with arguments a, built
@@ -1114,11 +1169,12 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
                 const TileArgs a = with_k(tile_args(h, cand[i].th, fdec), cand[i].k);
                 const int n = 8 / cand[i].k;
                 // the first pass of a shape loads its code: untimed
-                const uint32_t* ord = wave_order(h, cand[i].k, a, true);
-                if (rep == 0 && fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream, ord), "launch_tile")) break;
+                const OrderRef ord = wave_order(h, cand[i].k, a, true);
+                if (rep == 0 && fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream, ord.dev, ord.n), "launch_tile"))
+                    break;
                 if (fail_hip(hipEventRecord(ev[0], h->stream), "hipEventRecord")) break;
                 for (int l = 0; l < n && st == LGCA_OK; ++l)
-                    fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream, ord), "launch_tile");
+                    fail_hip(launch_tile(h->kind, cand[i].k, a, h->stream, ord.dev, ord.n), "launch_tile");
                 if (st || fail_hip(hipEventRecord(ev[1], h->stream), "hipEventRecord") ||
                     fail_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize"))
                     break;
@@ -1142,7 +1198,8 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
             for (uint32_t o = 1; o < 4 && k > 1 && st == LGCA_OK; ++o) {
                 TileArgs a = with_k(tile_args(h, h->tuned.th, fdec ? fdec + o * per_step : nullptr), k);
                 a.t0 += o;
-                fail_hip(launch_tile(h->kind, k, a, h->stream, wave_order(h, k, a, true)), "launch_tile");
+                const OrderRef ord = wave_order(h, k, a, true);
+                fail_hip(launch_tile(h->kind, k, a, h->stream, ord.dev, ord.n), "launch_tile");
             }
             if (st == LGCA_OK) fail_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         }
@@ -1338,7 +1395,8 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_d
                 h->trial_steps[ev_start] = 0;
             }
             const TileArgs a = with_k(tile_args(h, th, fdec), k);
-            HIPCHK(h, launch_tile(h->kind, k, a, h->stream, wave_order(h, k, a, ext_dec == nullptr)));
+            const OrderRef ord = wave_order(h, k, a, ext_dec == nullptr);
+            HIPCHK(h, launch_tile(h->kind, k, a, h->stream, ord.dev, ord.n));
             if (h->trial_open >= 0) h->trial_steps[h->trial_open] += (uint32_t)k;
             if (ev_end >= 0) {
                 HIPCHK(h, hipEventRecord(h->trial_ev[2 * ev_end + 1], h->stream));

@@ -68,14 +68,29 @@ bool tile_segments(const Geometry& g, int k);
 // modulo W) touch words [lo, hi) of those rows?
 bool tile_strip_hits(const Geometry& g, uint32_t stride, uint32_t halo, uint32_t st, int64_t lo, int64_t hi);
 // A pass of k steps; `order` is a dispatch order from tile_wave_order for the
-// same arguments and k, in device memory, or null.
-hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order = nullptr);
-// Strip-mode dispatch order of a pass (one entry per wave, tile * strips +
-// strip), from the rows' non-fluid word ranges row_lo / row_hi on the host
-// (null when the lattice has no solids).  False in segment mode, where the
-// order is fixed.
+// same arguments and k, in device memory, with n_order entries, or null.
+hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order = nullptr,
+                       uint32_t n_order = 0);
+// Strip-mode dispatch order of a pass, from the rows' non-fluid word ranges
+// row_lo / row_hi on the host (null when the lattice has no solids): one
+// entry per wave, (tile * strips + strip) << kPartBits | part, where part 0 is
+// the whole row tile and parts 1-2 and 3-6 its halves and quarters
+// (tile_part_rows).  Every row of every tile and strip is in exactly one
+// entry.  False in segment mode, where the order is fixed.
+constexpr uint32_t kPartBits = 3;
 bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int32_t* row_hi,
                      std::vector<uint32_t>& order);
+// Rows [off, off + n) of a tile of `rows` rows that part `part` covers: parts
+// of an even number of rows each (the last takes the rest), so that a part of
+// a tile that starts on an even row starts on one too.
+__host__ __device__ inline void tile_part_rows(uint32_t rows, uint32_t part, uint32_t& off, uint32_t& n)
+{
+    const uint32_t f = part == 0 ? 1u : part <= 2 ? 2u : 4u;  // parts of the tile
+    const uint32_t idx = part == 0 ? 0u : part <= 2 ? part - 1u : part - 3u;
+    const uint32_t q = (rows / f) & ~1u;
+    off = idx * q;
+    n = idx + 1 == f ? rows - off : q;
+}
 
 // Strip-mode wave order of tile_kernel (one wave per workgroup, one per row
 // tile and strip): entry blockIdx.x of `order` when there is one, otherwise
@@ -168,6 +183,11 @@ hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const W
                                 uint32_t n_items, uint32_t n_heavy, bool ragged, hipStream_t s);
 template <int KIND, int K>
 hipError_t launch_tile_pipe_kk(const TileArgs& a, const PipeArgs& p, uint32_t nwaves, hipStream_t s);
+template <int KIND, int K>
+int tile_waves_per_cu_kk(bool ragged);
+// Waves of the strip-mode tiled kernel of model `kind` and k steps per pass
+// that a CU holds at once, or 0 when unknown.
+int tile_waves_per_cu(int kind, int k, bool ragged);
 
 struct ForceDrawArgs {
     uint32_t* dec;              // [steps][fy1 - fy0][fwn]

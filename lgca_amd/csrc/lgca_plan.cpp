@@ -122,13 +122,28 @@ bool tile_wave_order(const TileArgs& a, int k, const int32_t* row_lo, const int3
     std::vector<uint8_t> cls;
     if (!wave_classes(a, k, row_lo, row_hi, nstrips, ntiles, cls)) return false;
     std::vector<uint32_t> solid, force, plain;
-    for (uint32_t it = 0; it < (uint32_t)cls.size(); ++it)
-        (cls[it] == 2 ? solid : cls[it] == 1 ? force : plain).push_back(it);
-    // The few long solid walks go first, spread over the XCDs; then the
-    // forced and the plain ones, each in contiguous runs per XCD, so that
-    // the tiles an XCD holds are vertical neighbours whose halo rows meet in
-    // its L2.
-    order.reserve((size_t)nstrips * ntiles);
+    for (uint32_t it = 0; it < (uint32_t)cls.size(); ++it) {
+        const uint32_t e = it << kPartBits;  // whole tile
+        if (cls[it] != 2) {
+            (cls[it] == 1 ? force : plain).push_back(e);
+            continue;
+        }
+        // A walk over solids runs up to 2.5 times as long as a plain one,
+        // long enough to outlast the pass: it is cut into quarters of the
+        // tile (halves on short tiles), each walking its rows plus the
+        // K-row halo, which costs more in all but ends sooner.
+        const uint32_t t = it / nstrips, rows = std::min(a.th, a.g.dim_y - t * a.th);
+        if (rows >= 16)
+            for (uint32_t p = 3; p <= 6; ++p) solid.push_back(e | p);
+        else if (rows >= 8)
+            for (uint32_t p = 1; p <= 2; ++p) solid.push_back(e | p);
+        else
+            solid.push_back(e);
+    }
+    // The solid walks go first, spread over the XCDs; then the forced and
+    // the plain ones, each in contiguous runs per XCD, so that the tiles an
+    // XCD holds are vertical neighbours whose halo rows meet in its L2.
+    order.reserve(solid.size() + force.size() + plain.size());
     order.insert(order.end(), solid.begin(), solid.end());
     append_xcd_runs(order, force);
     append_xcd_runs(order, plain);

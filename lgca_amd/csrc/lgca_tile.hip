@@ -39,7 +39,18 @@ hipError_t launch_tile_pipe(int kind, int k, const TileArgs& a, const PipeArgs& 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order)
+int tile_waves_per_cu(int kind, int k, bool ragged)
+{
+#define LGCA_OCC_CASE(KD, KK) \
+    if (kind == KD && k == KK) return tile_waves_per_cu_kk<KD, KK>(ragged);
+    LGCA_OCC_CASE(kHPP, 1) LGCA_OCC_CASE(kHPP, 2) LGCA_OCC_CASE(kHPP, 4) LGCA_OCC_CASE(kHPP, 8)
+    LGCA_OCC_CASE(kFHP1, 1) LGCA_OCC_CASE(kFHP1, 2) LGCA_OCC_CASE(kFHP1, 4) LGCA_OCC_CASE(kFHP1, 8)
+    LGCA_OCC_CASE(kFHP2, 1) LGCA_OCC_CASE(kFHP2, 2) LGCA_OCC_CASE(kFHP2, 4) LGCA_OCC_CASE(kFHP2, 8)
+#undef LGCA_OCC_CASE
+    return 0;
+}
+
+hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const uint32_t* order, uint32_t n_order)
 {
     const bool seg = tile_segments(a.g, k);
     if (!seg && (a.halo < tile_halo(a.g, k) || a.halo > 16)) return hipErrorInvalidValue;
@@ -57,6 +68,11 @@ hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const 
         if ((uint64_t)nstrips * ntiles >= ((uint64_t)1 << 31)) return hipErrorInvalidValue;
         nwaves = nstrips * ntiles;
         o.order = order;
+        if (order) {
+            // an order lists every tile and strip once, whole or in up to 4 parts
+            if ((uint64_t)n_order < nwaves || (uint64_t)n_order > 4ull * nwaves) return hipErrorInvalidValue;
+            nwaves = n_order;
+        }
         // without an order: the heavy strips are those over a forced word
         if (!order && a.fdec && a.fwn && nstrips <= 16384) {
             for (uint32_t st = 0; st < nstrips; ++st) {

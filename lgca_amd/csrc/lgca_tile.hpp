@@ -181,7 +181,7 @@ struct PipeWalk {
 };
 
 template <int KIND, int K, int O, bool RAGGED, bool SEG, bool ALT = false, bool PIPE = false>
-__device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile,
+__device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile, uint32_t part = 0,
                                           const PipeWalk* pw = nullptr)
 {
     constexpr int Q = num_dir_of_kind(KIND);
@@ -214,6 +214,12 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         if (strip >= nstrips) return;  // whole wave
         y0 = tile * a.th;
         rows = min(a.th, g.dim_y - y0);
+        if (part) {  // the part's rows of the tile (strip-mode orders only)
+            uint32_t off, n;
+            tile_part_rows(rows, part, off, n);
+            y0 += off;
+            rows = n;
+        }
         wabs = (int)(strip * stride) - (int)a.halo + (int)lane;
         writer = lane >= a.halo && lane < 64u - a.halo && wabs < (int)g.W;
         c.lsrc = c.rsrc = 0;
@@ -381,10 +387,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
         }
     }
 
-    auto walk = [&](auto side_tag, auto up_tag) {
-    constexpr bool SIDE = decltype(side_tag)::value;
+    // MODE 0: all fluid, unforced; 1: forced along x, no solids; 2: the
+    // general walk
+    auto walk = [&](auto mode_tag, auto up_tag) {
+    constexpr int MODE = decltype(mode_tag)::value;
     constexpr bool UP = decltype(up_tag)::value;
-    const bool t_solids = SIDE && tile_solids, t_force = SIDE && tile_force;
+    const bool t_solids = MODE == 2 && tile_solids;
+    const bool t_force = MODE == 1 || (MODE == 2 && tile_force);
     // pipeline windows between level b and b+1: rows j and the one before it
     // in walk order (j-1 walking down, j+1 walking up), of which only the
     // planes pulled across rows are kept (1, 2 down and 4, 5 up; HPP 1 or 3)
@@ -538,6 +547,9 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                     // fcur is zero outside the forced rows
                     if (t_force && (SEG || (j >= a.fy0 && j < a.fy1)))
                         apply_force<KIND, uint32_t>(st, fcur[s] & ~(ns | sl), a.force_dir);
+                } else if (MODE == 1) {
+                    // fcur is zero outside the forced rows and columns
+                    apply_force<KIND, uint32_t>(st, fcur[s], 1);
                 } else if (t_force && (SEG || (j >= a.fy0 && j < a.fy1))) {
                     apply_force<KIND, uint32_t>(st, fcur[s], a.force_dir);
                 }
@@ -619,19 +631,28 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     };  // walk
 
     // Most tiles of a large lattice are all fluid and unforced: they take a
-    // copy of the walk with that code left out (no tests, no branches).  Rows
-    // with a short last word keep the one walk: the second copy would push
-    // their kernels past the register budget.
-    const bool side = RAGGED || tile_solids || tile_force;
+    // copy of the walk with that code left out (no tests, no branches).  Tiles
+    // forced along x that hold no solids take a copy with the force applied
+    // on every level and none of the solid tests (in the general walk they
+    // ran 1.75 times as long as a plain walk).  Rows with a short last word,
+    // and the multi-pass kernel, keep fewer copies: more would push those
+    // kernels past the register budget.
+    using M0 = std::integral_constant<int, 0>;
+    using M1 = std::integral_constant<int, 1>;
+    using M2 = std::integral_constant<int, 2>;
+    const int mode = (RAGGED || tile_solids) ? 2 : !tile_force ? 0 : (a.force_dir == 1 && !PIPE) ? 1 : 2;
+    auto run = [&](auto up_tag) {
+        if (mode == 2) walk(M2{}, up_tag);
+        else if (mode == 1) walk(M1{}, up_tag);
+        else walk(M0{}, up_tag);
+    };
     if constexpr (ALT && !SEG) {
         if (up) {
-            if (side) walk(std::true_type{}, std::true_type{});
-            else walk(std::false_type{}, std::true_type{});
+            run(std::true_type{});
             return;
         }
     }
-    if (side) walk(std::true_type{}, std::false_type{});
-    else walk(std::false_type{}, std::false_type{});
+    run(std::false_type{});
 }
 
 // One wave per workgroup.  A CU takes a new workgroup only when every SIMD the
@@ -661,9 +682,11 @@ void tile_kernel(TileArgs a, TileOrder o)
     // mapped apart, so that every XCD gets its share of both (mapped as one
     // range, the forced waves all land on the first XCDs, which then finish
     // last).
-    uint32_t strip, tile;
+    uint32_t strip, tile, part = 0;
     if (o.order) {
-        const uint32_t it = o.order[blockIdx.x];
+        const uint32_t e = o.order[blockIdx.x];
+        const uint32_t it = e >> kPartBits;
+        part = e & ((1u << kPartBits) - 1u);
         tile = it / o.nstrips;
         strip = it - tile * o.nstrips;
     } else {
@@ -683,7 +706,7 @@ void tile_kernel(TileArgs a, TileOrder o)
                 if (o.heavy[h] <= strip) ++strip;
         }
     }
-    tile_body<KIND, K, O, RAGGED, false, true>(a, strip, tile);
+    tile_body<KIND, K, O, RAGGED, false, true>(a, strip, tile, part);
 }
 
 // Several passes in one launch (strip mode, K = 4 or 8, th >= K), so that the
@@ -777,7 +800,7 @@ void tile_pipe_kernel(TileArgs a, PipeArgs p)
         b.t0 = a.t0 + pass * (uint32_t)K;
         if (a.fdec) b.fdec = a.fdec + (size_t)pass * p.fdec_pass;
         pw.pass = pass;
-        tile_body<KIND, K, O, false, false, true, true>(b, strip, tile, &pw);
+        tile_body<KIND, K, O, false, false, true, true>(b, strip, tile, 0, &pw);
         // the pass's stores have all completed before the item says so
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         if (lane == 0)
@@ -879,6 +902,18 @@ hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves
 #undef LGCA_TILE_OFF
 #undef LGCA_TILE_LAUNCH
     return hipErrorInvalidValue;
+}
+
+// Waves of the strip-mode kernel of one (model, K) pair that a CU holds at
+// once (one wave per workgroup), or 0 when the runtime cannot say.
+template <int KIND, int K>
+int tile_waves_per_cu_kk(bool ragged)
+{
+    int n = 0;
+    const hipError_t e =
+        ragged ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tile_kernel<KIND, K, 0, true, false>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tile_kernel<KIND, K, 0, false, false>, 64, 0);
+    return e == hipSuccess ? n : 0;
 }
 
 // Pipelined launcher (K = 4 or 8, no short last word); lgca_tile.hip has

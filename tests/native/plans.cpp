@@ -1,9 +1,11 @@
 // Host-side launch planning of the tiled step (lgca_amd/csrc/lgca_plan.cpp),
 // checked without a GPU:
-//   * tile_wave_order lists every wave of a pass once; the walks over solid
-//     rows come before all others, the forced strips before the unforced
-//     ones, and a lattice without solids or forcing is handed out in
-//     contiguous ascending runs, one per XCD (entries b, b + 8, ... of the
+//   * tile_wave_order covers every row of every row tile and strip of a pass
+//     exactly once, whole tiles or parts of an even number of rows starting
+//     on even rows; the walks over solid rows come before all others (cut
+//     into parts on tiles of 8 rows or more), the forced strips before the
+//     unforced ones, and a lattice without solids or forcing is handed out
+//     in contiguous ascending runs, one per XCD (entries b, b + 8, ... of the
 //     order run on one XCD);
 //   * tile_pipe_plan cuts the row tiles into contiguous bands of whole tiles
 //     (at least two tiles and 2K rows each) that cover the lattice once, and
@@ -93,18 +95,46 @@ static void check_order(const Geometry& g, Shape shape, int k, uint32_t th, bool
     if (!ok) return;
 
     const uint32_t ns = tile_strips(g, k), nt = (g.dim_y + th - 1) / th;
-    std::vector<uint32_t> pos(order.size(), 0);
-    std::vector<uint8_t> seen((size_t)ns * nt, 0);
-    bool perm = order.size() == seen.size();
-    for (size_t i = 0; perm && i < order.size(); ++i) {
-        perm = order[i] < seen.size() && !seen[order[i]];
-        if (perm) {
-            seen[order[i]] = 1;
-            pos[order[i]] = (uint32_t)i;
-        }
+    // rows covered per wave (tile * ns + strip), and the positions of its
+    // first and last entries
+    std::vector<uint32_t> pos((size_t)ns * nt, UINT32_MAX), last((size_t)ns * nt, 0), covered((size_t)ns * nt, 0);
+    std::vector<uint8_t> parts((size_t)ns * nt, 0);
+    bool ok_parts = order.size() >= pos.size() && order.size() <= 4 * pos.size();
+    for (size_t i = 0; ok_parts && i < order.size(); ++i) {
+        const uint32_t w = order[i] >> kPartBits, part = order[i] & ((1u << kPartBits) - 1u);
+        ok_parts = w < pos.size() && part <= 6;
+        if (!ok_parts) break;
+        const uint32_t t = w / ns, rows = std::min(th, g.dim_y - t * th);
+        uint32_t off = 0, n = 0;
+        tile_part_rows(rows, part, off, n);
+        // parts of one split (halves or quarters) only, each even-sized,
+        // starting on an even row, and covering the tile once in all
+        const uint8_t kind = part == 0 ? 1 : part <= 2 ? 2 : 4;
+        ok_parts = (parts[w] == 0 || parts[w] == kind) && n > 0 && off % 2 == 0 &&
+                   (rows % 2 != 0 || n % 2 == 0) && off + n <= rows;
+        parts[w] = kind;
+        covered[w] += n;
+        pos[w] = std::min(pos[w], (uint32_t)i);
+        last[w] = std::max(last[w], (uint32_t)i);
     }
-    CHECK(perm, "order is a permutation (%u x %u, k %d, th %u, forced %d)", g.dim_x, g.dim_y, k, th, (int)forced);
-    if (!perm) return;
+    for (size_t w = 0; ok_parts && w < pos.size(); ++w)
+        ok_parts = covered[w] == std::min(th, g.dim_y - (uint32_t)(w / ns) * th);
+    CHECK(ok_parts, "order covers every row once (%u x %u, k %d, th %u, forced %d)", g.dim_x, g.dim_y, k, th,
+          (int)forced);
+    if (!ok_parts) return;
+    // every part of a split lies in the tile and the parts do not overlap:
+    // each part's rows are counted once above and their sum is the tile's
+    for (uint32_t rows = 2; rows <= 64; ++rows)
+        for (uint32_t f : {2u, 4u}) {
+            uint32_t next = 0;
+            for (uint32_t p = (f == 2 ? 1u : 3u); p < (f == 2 ? 3u : 7u); ++p) {
+                uint32_t off = 0, n = 0;
+                tile_part_rows(rows, p, off, n);
+                if (rows >= 4 * f) CHECK(off == next && n >= 2, "parts tile the rows (%u rows, part %u)", rows, p);
+                next = off + n;
+            }
+            if (rows >= 4 * f) CHECK(next == rows, "parts end at the tile end (%u rows)", rows);
+        }
 
     // solid walks first: every wave of a tile whose walk crosses a wall row
     // (all its strips cross solids) before every wave of a tile that crosses
@@ -121,10 +151,10 @@ static void check_order(const Geometry& g, Shape shape, int k, uint32_t th, bool
                           ((int64_t)t * th - k < 1 || (int64_t)(t + 1) * th + k > (int64_t)g.dim_y - 1);
         const bool clear = shape == kOpen || !walk_crosses(s, g.dim_y, th, t, k);
         for (uint32_t st = 0; st < ns; ++st) {
-            const uint32_t p = pos[(size_t)t * ns + st];
-            if (wall) last_wall = std::max(last_wall, p);
+            const uint32_t p = pos[(size_t)t * ns + st], q = last[(size_t)t * ns + st];
+            if (wall) last_wall = std::max(last_wall, q);
             if (clear) first_clear = std::min(first_clear, p);
-            if (clear && forced && st == 0) last_forced = std::max(last_forced, p);
+            if (clear && forced && st == 0) last_forced = std::max(last_forced, q);
             if (clear && forced && mid_clear && st == mid) first_mid = std::min(first_mid, p);
         }
     }
@@ -134,15 +164,20 @@ static void check_order(const Geometry& g, Shape shape, int k, uint32_t th, bool
         CHECK(last_forced < first_mid, "forced strips lead the clear tiles (%u x %u, k %d, th %u)", g.dim_x,
               g.dim_y, k, th);
 
+    // solid walks are cut into parts on tiles of 8 rows or more
+    if (shape != kOpen)
+        for (uint32_t st = 0; st < ns; ++st)
+            CHECK(parts[st] == (th >= 16 ? 4 : th >= 8 ? 2 : 1), "solid walk of tile 0 cut (%u x %u, k %d, th %u)",
+                  g.dim_x, g.dim_y, k, th);
     // one class: entries b, b + 8, b + 16, ... form an ascending run of
-    // consecutive waves, and the runs of XCDs 0..7 follow each other
+    // consecutive whole waves, and the runs of XCDs 0..7 follow each other
     if (shape == kOpen && !forced) {
         uint32_t next = 0;
         for (uint32_t x = 0; x < 8 && x < order.size(); ++x)
             for (size_t b = x; b < order.size(); b += 8) {
-                CHECK(order[b] == next, "contiguous XCD runs (%u x %u, k %d, th %u, entry %zu)", g.dim_x,
-                      g.dim_y, k, th, b);
-                next = order[b] + 1;
+                CHECK(order[b] == next << kPartBits, "contiguous XCD runs (%u x %u, k %d, th %u, entry %zu)",
+                      g.dim_x, g.dim_y, k, th, b);
+                next = (order[b] >> kPartBits) + 1;
             }
     }
 }
