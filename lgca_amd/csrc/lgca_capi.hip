@@ -121,6 +121,8 @@ struct lgca_lattice {
     uint32_t trial_group = 1;       // launches per timed group
     int trial_open = -1;            // group whose end event is still to come
     LaunchShape tuned{0, 0};        // the chosen shape, th 0 while undecided
+    uint32_t tuned_rem4 = 0;        // rows of the fastest 4-step candidate, for the
+                                    // 4-step remainders of an 8-step choice (0: rule)
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     // forcing controller of the viewers (lgca_controller_*)
@@ -177,6 +179,7 @@ void reset_trial(lgca_lattice* h)
     h->trial_issued = 0;
     h->trial_open = -1;
     h->tuned = LaunchShape{0, 0};
+    h->tuned_rem4 = 0;
 }
 
 template <typename T>
@@ -800,6 +803,28 @@ static uint32_t auto_tile_rows(const lgca_lattice* h, int k)
     return (uint32_t)th;
 }
 
+// Keeps candidate `pick` of a finished trial, and the fastest 4-step
+// candidate for the 4-step passes that finish a call of an 8-step choice.
+static void keep_trial_choice(lgca_lattice* h, const std::vector<LaunchShape>& cand,
+                              const std::vector<float>& best, size_t pick)
+{
+    h->tuned = cand[pick];
+    h->tuned_rem4 = 0;
+    float b4 = 1e30f;
+    for (size_t i = 0; i < cand.size(); ++i)
+        if (cand[i].k == 4 && best[i] < b4) {
+            b4 = best[i];
+            h->tuned_rem4 = cand[i].th;
+        }
+}
+
+// Rows per tile of a k-step pass that is not of the kept steps per pass.
+static uint32_t other_tile_rows(const lgca_lattice* h, int k)
+{
+    if (k == 4 && !h->tile_th && h->tuned.th && h->tuned_rem4) return h->tuned_rem4;
+    return auto_tile_rows(h, k);
+}
+
 // The largest steps per launch, at most `want` and `left`.  A pass may start
 // at any step: the tiled kernels are compiled for each offset of the first
 // step within its four-step chirality group.
@@ -941,12 +966,12 @@ static lgca_status plan_launch(lgca_lattice* h, uint64_t left, int* k, uint32_t*
             for (size_t c = 1; c < best.size(); ++c)
                 if (best[c] < best[pick]) pick = c;
             if (best[0] <= 1.02f * best[pick]) pick = 0;
-            h->tuned = h->trial[pick];
+            keep_trial_choice(h, h->trial, best, pick);
         }
     }
     if (h->tuned.th) {
         *k = steps_for_launch(h->tuned.k, left);
-        *th = *k == h->tuned.k ? h->tuned.th : auto_tile_rows(h, *k);
+        *th = *k == h->tuned.k ? h->tuned.th : other_tile_rows(h, *k);
         return LGCA_OK;
     }
     if (h->trial_issued < n) {
@@ -1191,15 +1216,20 @@ extern "C" lgca_status lgca_tune(lgca_lattice* h)
             h->trial.clear();
             h->trial_issued = 0;
             h->trial_open = -1;
-            h->tuned = cand[pick];
+            keep_trial_choice(h, cand, best, pick);
             // one untimed pass of the kept shape at each later offset within
-            // the four-step group, so that each of its kernels has run once
-            const int k = h->tuned.k;
-            for (uint32_t o = 1; o < 4 && k > 1 && st == LGCA_OK; ++o) {
-                TileArgs a = with_k(tile_args(h, h->tuned.th, fdec ? fdec + o * per_step : nullptr), k);
-                a.t0 += o;
-                const OrderRef ord = wave_order(h, k, a, true);
-                fail_hip(launch_tile(h->kind, k, a, h->stream, ord.dev, ord.n), "launch_tile");
+            // the four-step group, and of the shapes that finish calls whose
+            // steps are not a multiple of it (fewer steps per pass) at every
+            // offset, so that each kernel the following steps use has run once
+            // (a first launch loads code and takes longer)
+            for (int k = h->tuned.k; k >= 1 && st == LGCA_OK; k /= 2) {
+                const uint32_t th = k == h->tuned.k ? h->tuned.th : other_tile_rows(h, k);
+                for (uint32_t o = k == h->tuned.k ? 1 : 0; o < 4 && (k > 1 || o == 0) && st == LGCA_OK; ++o) {
+                    TileArgs a = with_k(tile_args(h, th, fdec ? fdec + o * per_step : nullptr), k);
+                    a.t0 += o;
+                    const OrderRef ord = wave_order(h, k, a, true);
+                    fail_hip(launch_tile(h->kind, k, a, h->stream, ord.dev, ord.n), "launch_tile");
+                }
             }
             if (st == LGCA_OK) fail_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         }
@@ -1298,7 +1328,7 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_d
         if (h->tile_k != 0 && ext_dec) {
             const LaunchShape sh = kept_shape(h);
             k = steps_for_launch(sh.k, left);
-            th = k == sh.k ? sh.th : auto_tile_rows(h, k);
+            th = k == sh.k ? sh.th : other_tile_rows(h, k);
         } else if (h->tile_k != 0) {
             lgca_status st = plan_launch(h, left, &k, &th, &ev_start, &ev_end);
             if (st) return st;
