@@ -145,6 +145,38 @@ def test_tile_strip_boundaries(Lattice, dx, k):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("model,dx,dy,geom,force,tuning", [
+    # forced strips without solids: the force-only walk, with decisions on
+    # some rows only (zero words elsewhere), every model
+    (0, 4096, 64, "periodic", dict(dir="x", p=0.2, x0=100, x1=300, y0=10, y1=50), (8, 16)),
+    (1, 4096, 64, "periodic", dict(dir="x", p=0.2, x0=100, x1=300, y0=10, y1=50), (8, 16)),
+    (2, 4096, 64, "periodic", dict(dir="x", p=0.2, x0=100, x1=300, y0=10, y1=50), (4, 16)),
+    (3, 4096, 64, "periodic", dict(dir="x", p=0.2, x0=4000, x1=4096, y0=10, y1=50), (8, 16)),
+    # a short last word keeps the general walk
+    (2, 4096 + 77, 64, "periodic", dict(dir="x", p=0.2, x0=100, x1=300, y0=10, y1=50), (8, 16)),
+    # walls and a cylinder: walks over solids in quarters (40 rows), halves
+    # (12 rows) and whole (6 rows), beside forced and plain walks
+    (3, 4096, 96, "cylinder", dict(dir="x", p=0.05, x0=0, x1=64), (8, 40)),
+    (1, 4096, 96, "cylinder", dict(dir="y", p=0.1), (4, 12)),
+    (2, 4096 + 5, 96, "cylinder", dict(dir="x", p=0.05, x0=0, x1=64), (8, 6)),
+    # forced everywhere: every strip forced, walls in the first and last tile
+    (2, 8192, 150, "pipe", dict(dir="x", p=0.01), (8, 74)),
+])
+def test_force_only_walks_and_solid_parts(Lattice, model, dx, dy, geom, force, tuning):
+    """Strip-mode passes whose waves take the force-only walk (forced along x,
+    no solids) or walk a quarter or half of a tile over solids: bits equal the
+    oracle's (src/omp_lattice.cpp:100-249, 254-346) whatever walk or part a
+    wave takes."""
+    rng = np.random.default_rng(dx * 7 + dy + model)
+    types = make_types(rng, dx, dy, geom)
+    state = random_state(rng, model, dx * dy)
+    say(f"[walks] model={model} {dx}x{dy} {geom} force={force} tuning={tuning}")
+    lat, got, want = run_both(Lattice, model, dx, dy, types, state, seed=dx + dy, step0=6,
+                              nsteps=29, force=force, chunks=2, tuning=tuning)
+    np.testing.assert_array_equal(got, want)
+    lat.close()
+
+
 def test_single_steps_match_batched_steps(Lattice):
     rng = np.random.default_rng(5)
     dx, dy = 160, 20
