@@ -175,9 +175,10 @@ hipError_t launch_tile_batch(int k, bool ragged, const BatchJob* jobs, BatchStat
 // Launchers of one (model, K) pair and of one K's batch kernel, instantiated
 // by lgca_tile_inst.hip; grid and shapes come checked from launch_tile /
 // launch_tile_batch.
+// seg: 0 strip mode, 1 segment mode, 2 segment mode with rows of 64 words
 template <int KIND, int K>
 hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves, bool ragged,
-                          bool seg, hipStream_t s);
+                          int seg, hipStream_t s);
 template <int K>
 hipError_t launch_tile_batch_kk(const BatchJob* jobs, BatchState* state, const WorkItem* items,
                                 uint32_t n_items, uint32_t n_heavy, bool ragged, hipStream_t s);

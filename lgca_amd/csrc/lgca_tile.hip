@@ -84,8 +84,11 @@ hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const 
         }
     }
     const bool ragged = a.g.nb != 32;
+    // segment mode with one row of exactly 64 words per wave: DPP rotations
+    // instead of permutes for the neighbouring words
+    const int seg_mode = !seg ? 0 : (a.g.W == 64 && !ragged) ? 2 : 1;
 #define LGCA_TILE_CASE(KD, KK) \
-    if (kind == KD && k == KK) return launch_tile_kk<KD, KK>(a, o, nwaves, ragged, seg, s);
+    if (kind == KD && k == KK) return launch_tile_kk<KD, KK>(a, o, nwaves, ragged, seg_mode, s);
     LGCA_TILE_CASE(kHPP, 1) LGCA_TILE_CASE(kHPP, 2) LGCA_TILE_CASE(kHPP, 4) LGCA_TILE_CASE(kHPP, 8)
     LGCA_TILE_CASE(kFHP1, 1) LGCA_TILE_CASE(kFHP1, 2) LGCA_TILE_CASE(kFHP1, 4) LGCA_TILE_CASE(kFHP1, 8)
     LGCA_TILE_CASE(kFHP2, 1) LGCA_TILE_CASE(kFHP2, 2) LGCA_TILE_CASE(kFHP2, 4) LGCA_TILE_CASE(kFHP2, 8)

@@ -92,17 +92,28 @@ __device__ __forceinline__ void bstore_coherent(uint32_t v, __amdgpu_buffer_rsrc
     __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, __builtin_amdgcn_readfirstlane(soff), kCoherent);
 }
 
+// Segment modes of the walk (template parameter SEG): strip mode, segment
+// mode (S = 64 / W rows per wave), and segment mode with rows of exactly 64
+// words, one per wave.
+constexpr int kStrips = 0, kSegments = 1, kWholeRow = 2;
+
 // Neighbouring words: the adjacent lane (DPP) when a wave holds consecutive
-// words of one row, or the lane holding word w -+ 1 of the same row in
-// segment mode, where the row wraps inside its segment (ds_bpermute).
-template <bool SEG>
+// words of one row; in segment mode, where the row wraps inside its
+// segment, the lane holding word w -+ 1 of the same row (ds_bpermute), or,
+// when the segment is the whole wave, the rotation of the wave by one lane
+// (DPP wave_ror:1 / wave_rol:1).  A permute goes through the LDS crossbar and
+// its result is waited for at every shift: walks of small lattices, one wave
+// per SIMD, wait it out four times per level.
+template <int SEG>
 __device__ __forceinline__ uint32_t left_word(uint32_t v, const LaneCtx& c)
 {
+    if (SEG == kWholeRow) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x13C, 0xF, 0xF, true);
     return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.lsrc, (int)v) : from_left(v);
 }
-template <bool SEG>
+template <int SEG>
 __device__ __forceinline__ uint32_t right_word(uint32_t v, const LaneCtx& c)
 {
+    if (SEG == kWholeRow) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x134, 0xF, 0xF, true);
     return SEG ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)c.rsrc, (int)v) : from_right(v);
 }
 
@@ -134,7 +145,7 @@ __host__ __device__ constexpr int chir_src(int O, int s)
 __host__ __device__ constexpr int chir_delay(int O, int s) { return s - chir_src(O, s); }
 
 // out cell x = in cell x-1, cyclically over dim_x
-template <bool RAGGED, bool SEG>
+template <bool RAGGED, int SEG>
 __device__ __forceinline__ uint32_t shift_m1(uint32_t v, const LaneCtx& c)
 {
     uint32_t L = left_word<SEG>(v, c);
@@ -142,7 +153,7 @@ __device__ __forceinline__ uint32_t shift_m1(uint32_t v, const LaneCtx& c)
     return __builtin_amdgcn_alignbit(v, L, 31);
 }
 // out cell x = in cell x+1, cyclically over dim_x
-template <bool RAGGED, bool SEG>
+template <bool RAGGED, int SEG>
 __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 {
     const uint32_t R = right_word<SEG>(v, c);
@@ -156,10 +167,10 @@ __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 // K <= 4 fits 128 VGPRs (four waves per SIMD); K = 8 needs more.
 template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
 
-// One wave's work.  Strip mode (SEG false): x-strip `strip` of row tile
-// `tile`.  Segment mode (SEG true, rows of W <= 64 words): the wave's lanes
-// form S = 64 / W segments of W lanes, each holding a whole row; segment s
-// walks row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when
+// One wave's work.  Strip mode (SEG kStrips): x-strip `strip` of row tile
+// `tile`.  Segment mode (SEG kSegments or kWholeRow, rows of W <= 64 words):
+// the wave's lanes form S = 64 / W segments of W lanes, each holding a whole
+// row; segment s walks row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when
 // K == 1, whose one level picks its word at run time), or kRuntimeOffset.
 //
 // ALT (strip mode): odd row tiles walk up the lattice, even ones down.  Two
@@ -180,7 +191,7 @@ struct PipeWalk {
     uint32_t* exch;         // [pass & 1][band][0 first / 1 last][K rows][Q planes][P words]
 };
 
-template <int KIND, int K, int O, bool RAGGED, bool SEG, bool ALT = false, bool PIPE = false>
+template <int KIND, int K, int O, bool RAGGED, int SEG, bool ALT = false, bool PIPE = false>
 __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uint32_t tile, uint32_t part = 0,
                                           const PipeWalk* pw = nullptr)
 {
@@ -667,13 +678,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
 // the grid; so they are dispatched first (tile_wave_order; without an order,
 // the forced strips of every row tile) and the light ones fill in behind
 // them.
-template <int KIND, int K, int O, bool RAGGED, bool SEG>
+template <int KIND, int K, int O, bool RAGGED, int SEG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TileOccupancy<K>::waves, 8)))
 void tile_kernel(TileArgs a, TileOrder o)
 {
     // segment mode: grid (tile groups); strip mode: grid (nstrips * ntiles)
     if (SEG) {
-        tile_body<KIND, K, O, RAGGED, true>(a, 0, blockIdx.x);
+        tile_body<KIND, K, O, RAGGED, SEG>(a, 0, blockIdx.x);
         return;
     }
     // Workgroups go to the 8 XCDs round-robin.  Each XCD takes a contiguous
@@ -706,7 +717,7 @@ void tile_kernel(TileArgs a, TileOrder o)
                 if (o.heavy[h] <= strip) ++strip;
         }
     }
-    tile_body<KIND, K, O, RAGGED, false, true>(a, strip, tile, part);
+    tile_body<KIND, K, O, RAGGED, kStrips, true>(a, strip, tile, part);
 }
 
 // Several passes in one launch (strip mode, K = 4 or 8, th >= K), so that the
@@ -800,7 +811,7 @@ void tile_pipe_kernel(TileArgs a, PipeArgs p)
         b.t0 = a.t0 + pass * (uint32_t)K;
         if (a.fdec) b.fdec = a.fdec + (size_t)pass * p.fdec_pass;
         pw.pass = pass;
-        tile_body<KIND, K, O, false, false, true, true>(b, strip, tile, 0, &pw);
+        tile_body<KIND, K, O, false, kStrips, true, true>(b, strip, tile, 0, &pw);
         // the pass's stores have all completed before the item says so
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         if (lane == 0)
@@ -849,14 +860,14 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
     const bool seg = (it.shape >> 31) != 0;
     const uint32_t Q = __builtin_amdgcn_readfirstlane(a.g.Q);
     if (Q == 4) {
-        if (seg) tile_body<kHPP, K, O, RAGGED, true>(a, it.strip, it.tile);
-        else tile_body<kHPP, K, O, RAGGED, false>(a, it.strip, it.tile);
+        if (seg) tile_body<kHPP, K, O, RAGGED, kSegments>(a, it.strip, it.tile);
+        else tile_body<kHPP, K, O, RAGGED, kStrips>(a, it.strip, it.tile);
     } else if (Q == 6) {
-        if (seg) tile_body<kFHP1, K, O, RAGGED, true>(a, it.strip, it.tile);
-        else tile_body<kFHP1, K, O, RAGGED, false>(a, it.strip, it.tile);
+        if (seg) tile_body<kFHP1, K, O, RAGGED, kSegments>(a, it.strip, it.tile);
+        else tile_body<kFHP1, K, O, RAGGED, kStrips>(a, it.strip, it.tile);
     } else {
-        if (seg) tile_body<kFHP2, K, O, RAGGED, true>(a, it.strip, it.tile);
-        else tile_body<kFHP2, K, O, RAGGED, false>(a, it.strip, it.tile);
+        if (seg) tile_body<kFHP2, K, O, RAGGED, kSegments>(a, it.strip, it.tile);
+        else tile_body<kFHP2, K, O, RAGGED, kStrips>(a, it.strip, it.tile);
     }
     // The job's last wave of the round advances it.  Every other wave of the
     // job has read the state and counted itself, so nothing in this launch
@@ -876,22 +887,23 @@ void tile_batch_kernel(const BatchJob* __restrict__ jobs, BatchState* state,
 // built the grid.  The pass offset O = t0 & 3 selects the instantiation.
 template <int KIND, int K>
 hipError_t launch_tile_kk(const TileArgs& a, const TileOrder& o, uint32_t nwaves, bool ragged,
-                          bool seg, hipStream_t s)
+                          int seg, hipStream_t s)
 {
     const int off = K == 1 ? 0 : (int)(a.t0 & 3u);
     const dim3 grid(nwaves), block(64);
 #define LGCA_TILE_LAUNCH(OO, RG, SG) \
     hipLaunchKernelGGL((tile_kernel<KIND, K, OO, RG, SG>), grid, block, 0, s, a, o)
-#define LGCA_TILE_OFF(OO)                                         \
-    if (off == OO) {                                              \
-        if (ragged) {                                             \
-            if (seg) LGCA_TILE_LAUNCH(OO, true, true);            \
-            else LGCA_TILE_LAUNCH(OO, true, false);               \
-        } else {                                                  \
-            if (seg) LGCA_TILE_LAUNCH(OO, false, true);           \
-            else LGCA_TILE_LAUNCH(OO, false, false);              \
-        }                                                         \
-        return hipGetLastError();                                 \
+#define LGCA_TILE_OFF(OO)                                                   \
+    if (off == OO) {                                                        \
+        if (ragged) {                                                       \
+            if (seg) LGCA_TILE_LAUNCH(OO, true, kSegments);                 \
+            else LGCA_TILE_LAUNCH(OO, true, kStrips);                       \
+        } else {                                                            \
+            if (seg == kWholeRow) LGCA_TILE_LAUNCH(OO, false, kWholeRow);   \
+            else if (seg) LGCA_TILE_LAUNCH(OO, false, kSegments);           \
+            else LGCA_TILE_LAUNCH(OO, false, kStrips);                      \
+        }                                                                   \
+        return hipGetLastError();                                           \
     }
     LGCA_TILE_OFF(0)
     if constexpr (K > 1) {
@@ -911,8 +923,8 @@ int tile_waves_per_cu_kk(bool ragged)
 {
     int n = 0;
     const hipError_t e =
-        ragged ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tile_kernel<KIND, K, 0, true, false>, 64, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tile_kernel<KIND, K, 0, false, false>, 64, 0);
+        ragged ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tile_kernel<KIND, K, 0, true, kStrips>, 64, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, tile_kernel<KIND, K, 0, false, kStrips>, 64, 0);
     return e == hipSuccess ? n : 0;
 }
 

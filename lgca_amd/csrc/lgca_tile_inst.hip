@@ -13,7 +13,7 @@ namespace lgca {
 
 #if LGCA_INST_KIND >= 0
 template hipError_t launch_tile_kk<LGCA_INST_KIND, LGCA_INST_K>(const TileArgs&, const TileOrder&, uint32_t,
-                                                                bool, bool, hipStream_t);
+                                                                bool, int, hipStream_t);
 template int tile_waves_per_cu_kk<LGCA_INST_KIND, LGCA_INST_K>(bool);
 #if LGCA_INST_K >= 4
 template hipError_t launch_tile_pipe_kk<LGCA_INST_KIND, LGCA_INST_K>(const TileArgs&, const PipeArgs&, uint32_t,

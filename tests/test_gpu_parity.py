@@ -110,6 +110,10 @@ STEP_CASES = [
     (2, 256, 200, "sparse", None),
     (1, 1024, 130, "sparse", dict(dir="y", p=0.2, y0=100, y1=130)),
     (2, 4096, 200, "sparse", dict(dir="x", p=0.1, y0=60, y1=80)),
+    # rows of exactly 64 words: one row per wave, neighbours by wave rotation
+    (0, 2048, 10, "mixed", dict(dir="y", p=0.3)),
+    (1, 2048, 12, "cylinder", dict(dir="x", p=0.2, x0=0, x1=100)),
+    (3, 2048, 8, "slipbox", None),
 ]
 
 
