@@ -179,11 +179,15 @@ LGCA_API lgca_status lgca_set_tuning(lgca_lattice* h, int steps_per_pass, uint32
 LGCA_API lgca_status lgca_set_pipeline(lgca_lattice* h, int on);
 /* Runs that trial now and waits for it: every candidate shape is timed on
  * passes that read the live state and write the other state buffer, and the
- * fastest per step is kept (lgca_get_info reports it).  The state and the
- * step do not change.  Does nothing when lgca_set_tuning fixed the rows per
- * tile or selected the one-step kernel.  No counterpart in the reference;
- * benchmarks call it before their warm-up, so that timed steps run the kept
- * shape from the first. */
+ * fastest per step is kept (lgca_get_info reports it), with the fastest
+ * 4-step shape for the shorter passes that finish calls of an 8-step choice.
+ * Candidates include tile heights whose passes fill the GPU's wave slots
+ * once or twice.  Each kernel the kept shapes use then runs once untimed, at
+ * every offset of the first step within its four-step group.  The state and
+ * the step do not change.  Does nothing when lgca_set_tuning fixed the rows
+ * per tile or selected the one-step kernel.  No counterpart in the
+ * reference; benchmarks call it before their warm-up, so that timed steps
+ * run the kept shapes from the first. */
 LGCA_API lgca_status lgca_tune(lgca_lattice* h);
 
 /* Count-mode body force: replaces apply_body_force(int forcing)
