@@ -386,13 +386,13 @@ def test_codec_round_trip_ragged(Lattice):
 
 def test_full_karman_size_through_the_default_path(Lattice):
     """FHP-III 32768x8192 with the cylinder of diameter 512 at (4096, 4099),
-    walls, fill 0.25, x force p = 2e-3 on x < 64, seed 11, r = 32: 240 steps
+    walls, fill 0.25, x force p = 2e-3 on x < 64, seed 11, r = 32: 256 steps
     through the default launch path (the trial's candidate shapes, two
     passes each: 4-step passes of 7 heights, 8-step passes of 6 fixed heights
-    and of up to 4 sized to the GPU's wave slots, 200 steps at most; then the
+    and of up to 4 sized to the GPU's wave slots, 216 steps at most; then the
     kept shape) equal the oracle's bits, and so do the coarse fields of the
     final state."""
-    dx, dy, steps = 32768, 8192, 240
+    dx, dy, steps = 32768, 8192, 256
     say("[full] karman 32768x8192")
     lat = Lattice(3, dim_x=dx, dim_y=dy, coarse_graining_radius=32, seed=11)
     lat.apply_bc_cylinder(4096, 4099, 512.0)
@@ -404,7 +404,7 @@ def test_full_karman_size_through_the_default_path(Lattice):
     np.testing.assert_array_equal(state, want0)
     del want0
     n0 = lat.get_n_particles()
-    lat.step(200)
+    lat.step(216)
     lat.synchronize()
     lat.step(40)  # decides on the trial, then runs the kept shape
     info = lat.info()
