@@ -168,10 +168,11 @@ __device__ __forceinline__ uint32_t shift_p1(uint32_t v, const LaneCtx& c)
 template <int K> struct TileOccupancy { static constexpr int waves = K <= 4 ? 4 : 2; };
 
 // One wave's work.  Strip mode (SEG kStrips): x-strip `strip` of row tile
-// `tile`.  Segment mode (SEG kSegments or kWholeRow, rows of W <= 64 words):
-// the wave's lanes form S = 64 / W segments of W lanes, each holding a whole
-// row; segment s walks row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when
-// K == 1, whose one level picks its word at run time), or kRuntimeOffset.
+// `tile`, or part `part` of it (tile_part_rows).  Segment mode (SEG
+// kSegments or kWholeRow, rows of W <= 64 words): the wave's lanes form
+// S = 64 / W segments of W lanes, each holding a whole row; segment s walks
+// row tile tile * S + s, and `strip` is unused.  O = a.t0 & 3 (0 when K == 1,
+// whose one level picks its word at run time), or kRuntimeOffset.
 //
 // ALT (strip mode): odd row tiles walk up the lattice, even ones down.  Two
 // vertically adjacent tiles then read the K halo rows they share at the same
