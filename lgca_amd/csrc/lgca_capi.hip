@@ -902,8 +902,13 @@ static std::vector<LaunchShape> trial_candidates(const lgca_lattice* h)
     };
     for (uint32_t r : {4u, 8u, 12u, 16u, 20u, 24u, 32u}) add(base_k, r);
     if (h->tile_k == kAutoSteps && c[0].th == 2) {
+        // small lattices: their walks are short chains of dependent steps, and
+        // fewer or more steps per launch can win (the 2048 x 512 pipe keeps
+        // 2-step passes, the 256 x 256 HPP lattice 8-step ones: 0.84 against
+        // 1.14 us per step at 4)
         add(2, 2);
         add(2, 4);
+        add(8, 2);
     }
     if (h->tile_k == kAutoSteps && c[0].th >= 4) {
         for (uint32_t r : {16u, 24u, 32u, 40u, 48u, 64u}) add(8, r);

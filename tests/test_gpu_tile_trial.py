@@ -42,9 +42,9 @@ def test_trial_picks_a_candidate_and_keeps_results():
     ref.set_tuning(4, 20)
     assert lat.info().tile_rows == 0
     assert lat.info().steps_per_pass == -1
-    # 8 candidates x 2 groups x 4 launches x 4 steps, and 2 x 2 x 4 x 2 steps
-    # at K = 2 (this lattice's rule gives 2-row tiles): 288 steps of trial, in
-    # uneven calls
+    # 8 candidates x 2 groups x 4 launches x 4 steps, 2 x 2 x 4 x 2 steps at
+    # K = 2 and 2 x 4 x 8 at K = 8 (this lattice's rule gives 2-row tiles):
+    # 352 steps of trial, in uneven calls
     for n in (5, 3, 17, 9, 30, 1, 22, 13, 7, 11, 101, 41, 161):
         lat.step(n)
         ref.step(n)
@@ -53,7 +53,7 @@ def test_trial_picks_a_candidate_and_keeps_results():
     ref.step(4)
     rows, k = lat.info().tile_rows, lat.info().steps_per_pass
     say(f"[trial] chosen tile rows {rows}, steps per pass {k}")
-    assert (k == 4 and rows in CANDIDATES) or (k == 2 and rows in {2, 4})
+    assert (k == 4 and rows in CANDIDATES) or (k == 2 and rows in {2, 4}) or (k, rows) == (8, 2)
     np.testing.assert_array_equal(lat.get_state_u8(), ref.get_state_u8())
     assert lat.steps == ref.steps == 425
     # and both agree with the oracle
@@ -75,7 +75,7 @@ def test_trial_on_a_lattice_shorter_than_its_candidates():
     lat = Lattice(2, dim_x=1024, dim_y=14, coarse_graining_radius=1, seed=3)
     lat.apply_bc_periodic()
     lat.init_random(0.3)
-    lat.step(200)
+    lat.step(240)  # the trial: 224 steps
     lat.synchronize()
     lat.step(4)
     assert lat.info().tile_rows in {2, 4, 8, 12}
@@ -85,8 +85,9 @@ def test_trial_on_a_lattice_shorter_than_its_candidates():
 
 def test_automatic_steps_per_pass_on_a_small_lattice():
     """The 2048 x 512 pipe (forced, walls): the default shape trial also times
-    2-step passes; whatever it keeps, results equal a fixed shape's and the
-    oracle's, and an explicit steps_per_pass is kept."""
+    2-step passes and 8-step passes of 2-row tiles (352 steps); whatever it
+    keeps, results equal a fixed shape's and the oracle's, and an explicit
+    steps_per_pass is kept."""
     from lgca_amd import Lattice
     dx, dy, seed = 2048, 512, 7
 
@@ -102,7 +103,7 @@ def test_automatic_steps_per_pass_on_a_small_lattice():
     types, state = lat.m_cell_type_cpu.copy(), lat.get_state_u8()
     np.testing.assert_array_equal(state, ref.get_state_u8())
     say(f"[trial] {dx}x{dy} FHP-II pipe, automatic steps per pass")
-    for n in (3, 150, 1, 29, 164):
+    for n in (3, 150, 1, 29, 200):
         lat.step(n)
         ref.step(n)
     lat.synchronize()
@@ -110,10 +111,10 @@ def test_automatic_steps_per_pass_on_a_small_lattice():
     ref.step(6)
     info = lat.info()
     say(f"[trial] chosen steps per pass {info.steps_per_pass}, tile rows {info.tile_rows}")
-    assert (info.steps_per_pass, info.tile_rows) in {(4, r) for r in CANDIDATES} | {(2, 2), (2, 4)}
+    assert (info.steps_per_pass, info.tile_rows) in {(4, r) for r in CANDIDATES} | {(2, 2), (2, 4), (8, 2)}
     got = lat.get_state_u8()
     np.testing.assert_array_equal(got, ref.get_state_u8())
-    want = orc.run(2, dx, dy, types, state, seed, 0, 353, dict(dir="x", p=1e-3, x0=0, x1=dx), nthreads=8)
+    want = orc.run(2, dx, dy, types, state, seed, 0, 389, dict(dir="x", p=1e-3, x0=0, x1=dx), nthreads=8)
     np.testing.assert_array_equal(got, want)
     # an explicit steps per pass keeps the trial to heights at that K
     lat.set_tuning(4, 0)
