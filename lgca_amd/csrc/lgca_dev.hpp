@@ -196,6 +196,20 @@ __host__ __device__ __forceinline__ void apply_solids(T c[7], const T in[7], T n
     for (int d = 0; d < Q; ++d) c[d] = o[d];
 }
 
+// apply_solids where no cell of the word is SLIP (sl == 0): a NO_SLIP bit
+// takes the bounce-back of the pulled value, a fluid bit the collided value,
+// one three-input select per plane.
+template <int KIND, typename T>
+__host__ __device__ __forceinline__ void apply_no_slip(T c[7], const T in[7], T ns)
+{
+    constexpr int Q = num_dir_of_kind(KIND);
+    T o[7];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) o[d] = bit3<kMux>(ns, in[inv_dir<KIND>(d)], c[d]);
+#pragma unroll
+    for (int d = 0; d < Q; ++d) c[d] = o[d];
+}
+
 // Body-force flip rules of apply_body_force (src/omp_lattice.cpp:293-337) on
 // the cells selected by f (already restricted to FLUID cells).
 template <int KIND, typename T>

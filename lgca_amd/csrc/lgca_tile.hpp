@@ -554,8 +554,14 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                 collide<KIND, uint32_t>(st, p);
                 if (t_solids) {
                     const uint32_t ns = mns[s], sl = msl[s];
-                    if (__builtin_amdgcn_ballot_w64((ns | sl) != 0) != 0)
-                        apply_solids<KIND, uint32_t>(st, in, ns, sl, edge, j == 0 || j + 1 == g.dim_y);
+                    if (__builtin_amdgcn_ballot_w64((ns | sl) != 0) != 0) {
+                        // rows of NO_SLIP cells only (walls, obstacles) take
+                        // the one-select rule
+                        if (__builtin_amdgcn_ballot_w64(sl != 0) == 0)
+                            apply_no_slip<KIND, uint32_t>(st, in, ns);
+                        else
+                            apply_solids<KIND, uint32_t>(st, in, ns, sl, edge, j == 0 || j + 1 == g.dim_y);
+                    }
                     // fcur is zero outside the forced rows
                     if (t_force && (SEG || (j >= a.fy0 && j < a.fy1)))
                         apply_force<KIND, uint32_t>(st, fcur[s] & ~(ns | sl), a.force_dir);
