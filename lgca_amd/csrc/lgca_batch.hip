@@ -496,6 +496,13 @@ std::string validate(const lgca_job_desc& d)
         return "force_dir must be NONE, X or Y";
     if (d.force_dir != LGCA_FORCE_NONE && !(d.force_p >= 0.0 && d.force_p <= 1.0))
         return "force_p must lie in [0, 1]";
+    if (d.force_dir != LGCA_FORCE_NONE) {
+        // a pass reads its steps' decisions through one buffer descriptor
+        const uint64_t x1 = std::min(d.fx1, d.dim_x), y1 = std::min(d.fy1, d.dim_y);
+        if (d.fx0 < x1 && d.fy0 < y1 &&
+            (y1 - d.fy0) * ((x1 + 31) / 32 - d.fx0 / 32) * 8u * 4u >= (1ull << 31))
+            return "force region too large (the decisions of an 8-step pass must stay under 2 GiB)";
+    }
     return "";
 }
 

@@ -893,9 +893,15 @@ static uint32_t fill_tile_rows(const lgca_lattice* h, int k, uint64_t slots)
 static std::vector<LaunchShape> trial_candidates(const lgca_lattice* h)
 {
     const int base_k = h->tile_k == kAutoSteps ? 4 : h->tile_k;
-    std::vector<LaunchShape> c(1, LaunchShape{base_k, auto_tile_rows(h, base_k)});
+    auto too_big = [&](int ck) {  // see lgca_step: decisions of a pass over 2 GiB
+        return h->force_dir != LGCA_FORCE_NONE &&
+               (uint64_t)ck * decision_words_per_step(h) * 4u >= ((uint64_t)1 << 31);
+    };
+    int k0 = base_k;
+    while (k0 > 1 && too_big(k0)) k0 /= 2;
+    std::vector<LaunchShape> c(1, LaunchShape{k0, auto_tile_rows(h, k0)});
     auto add = [&](int ck, uint32_t cth) {
-        if (cth > h->dim_y) return;
+        if (cth > h->dim_y || too_big(ck)) return;  // too big: cut by lgca_step, not timed
         for (const LaunchShape& s : c)
             if (s.k == ck && s.th == cth) return;
         c.push_back(LaunchShape{ck, cth});
@@ -1321,6 +1327,7 @@ extern "C" lgca_status lgca_step_noalloc(lgca_lattice* h, uint64_t nsteps)
 // nothing is allocated.
 static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_dec, size_t ext_words)
 {
+    constexpr uint64_t kMaxPassDecisionBytes = (uint64_t)1 << 31;
     const bool forcing = h->force_dir != LGCA_FORCE_NONE;
     const uint32_t fw0 = h->fx0 / 32, fw1 = (h->fx1 + 31) / 32;
     const size_t per_step = decision_words_per_step(h);
@@ -1337,6 +1344,16 @@ static lgca_status step_locked(lgca_lattice* h, uint64_t nsteps, uint32_t* ext_d
         } else if (h->tile_k != 0) {
             lgca_status st = plan_launch(h, left, &k, &th, &ev_start, &ev_end);
             if (st) return st;
+        }
+        // A pass reads its k steps of force decisions through one buffer
+        // descriptor of under 2 GiB: lattices whose forced region is larger
+        // than that per pass (some 2 G forced cells at 8 steps) take fewer
+        // steps per pass (a trial launch so cut is timed with its group).
+        if (forcing && h->tile_k != 0 && (uint64_t)k * per_step * 4u >= kMaxPassDecisionBytes) {
+            while (k > 1 && (uint64_t)k * per_step * 4u >= kMaxPassDecisionBytes) k /= 2;
+            if ((uint64_t)k * per_step * 4u >= kMaxPassDecisionBytes)
+                return fail(h, LGCA_ERR_INVALID_ARGUMENT, "lgca_step: the force region's decisions of one step exceed 2 GiB");
+            th = other_tile_rows(h, k);
         }
         // Several passes of the kept shape in one launch: all the steps left, as
         // far as one window of force decisions reaches (not while the trial

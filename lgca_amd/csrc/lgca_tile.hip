@@ -55,6 +55,9 @@ hipError_t launch_tile(int kind, int k, const TileArgs& a, hipStream_t s, const 
     const bool seg = tile_segments(a.g, k);
     if (!seg && (a.halo < tile_halo(a.g, k) || a.halo > 16)) return hipErrorInvalidValue;
     if (a.th < 2 || (a.th & 1u) || a.th > 65535) return hipErrorInvalidValue;
+    // the pass's k steps of force decisions go through one buffer descriptor,
+    // with room past its end for the offset that reads 0
+    if (a.fdec && (uint64_t)k * a.fdec_stride * 4u >= ((uint64_t)1 << 31)) return hipErrorInvalidValue;
     const uint32_t ntiles = (a.g.dim_y + a.th - 1) / a.th;
     uint32_t nwaves = 0;  // one wave per workgroup
     TileOrder o{};

@@ -283,6 +283,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
     // waits for those loads would then wait for the stores as well.
     constexpr uint32_t kOutside = 0x80000000u;
     const uint32_t fplane_b = (a.fy1 - a.fy0) * a.fwn * 4u;  // one step's force decisions
+    // the pass's K steps of decisions, one after another (a.fdec_stride is
+    // one step's words): one descriptor for every level, the level's step in
+    // the scalar offset
+    const uint32_t fall_b = (uint32_t)K * fplane_b;
     auto store_plane = [&](uint32_t r, int d, uint32_t v, bool wr) {
         if (SEG) bstore(v, make_rsrc(dst, g.dim_y * row_b), wr ? r * row_b + c.wb : kOutside, (uint32_t)d * plane_b);
         else bstore(v, make_rsrc(dst + (size_t)r * Q * g.P, row_b), wr ? c.wb : kOutside, (uint32_t)d * plane_b);
@@ -482,16 +486,16 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, uint32_t strip, uin
                     // next iteration, level s works on the row level s - 1 works on now
                     const uint32_t jn = s == 1 ? row_i : lrow[s - 1];
                     const bool lev = STEADY || i + 1 >= (uint32_t)(2 * s);
-                    const uint32_t* base = a.fdec + (size_t)(s - 1) * a.fdec_stride;
+                    const __amdgpu_buffer_rsrc_t fr = make_rsrc(a.fdec, fall_b);
+                    const uint32_t fstep = (uint32_t)(s - 1) * fplane_b;
                     if (SEG) {  // rows differ per lane
                         const bool ok = lev && jn >= a.fy0 && jn < a.fy1 && force_mine;
                         const uint32_t voff = ok ? ((jn - a.fy0) * a.fwn + (c.w - a.fw0)) * 4u : kOutside;
-                        fnext[s] = bload(make_rsrc(base, fplane_b), voff, 0);
-                    } else {    // the row is uniform: its offset is scalar, an
-                                // empty descriptor reads 0 for rows outside
+                        fnext[s] = bload(fr, voff, fstep);
+                    } else {    // the row is uniform: its offset is scalar; an
+                                // offset past the buffer reads 0 for rows outside
                         const bool ok = lev && jn >= a.fy0 && jn < a.fy1;
-                        fnext[s] = bload(make_rsrc(base, ok ? fplane_b : 0u), fvoff,
-                                         ok ? (jn - a.fy0) * a.fwn * 4u : 0u);
+                        fnext[s] = bload(fr, ok ? fvoff : kOutside, fstep + (ok ? (jn - a.fy0) * a.fwn * 4u : 0u));
                     }
                 }
             }

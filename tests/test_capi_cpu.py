@@ -123,3 +123,19 @@ def test_ctypes_structures_match_the_c_layout(tmp_path):
         want = C.sizeof(py) if field == "size" else getattr(py, field).offset
         assert int(value) == want, line
 
+
+
+def test_batch_refuses_force_regions_past_one_descriptor():
+    """A pass reads its 8 steps of force decisions through one buffer
+    descriptor: the batch refuses forced regions whose decisions of an
+    8-step pass reach 2 GiB (rows x 32-cell words x 8 steps x 4 bytes), and
+    accepts the same lattice with a smaller region."""
+    from lgca_amd.batch import Job, job_bytes
+    dy = 65534
+    ok = Job("FHP_II", 32 * 1024, dy, 10, force_dir="x", force_p=0.01)   # 2,147,418,112 bytes
+    assert job_bytes(ok) > 0
+    big = Job("FHP_II", 32 * 1025, dy, 10, force_dir="x", force_p=0.01)  # 2,149,515,520 bytes
+    with pytest.raises(ValueError):
+        job_bytes(big)
+    small = Job("FHP_II", 32 * 1025, dy, 10, force_dir="x", force_p=0.01, force_region=(0, 64, 0, dy))
+    assert job_bytes(small) > 0
